@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark: plain fp32 MLEM iterations on one MI355X.
+
+A step is one MLEM iteration (forward projection + ratio, backprojection +
+update) over the whole sinogram. The workload is the 128x128x48-grid,
+1,327,104-line system matrix (Siddon lengths, seeds 1001/2001/3001, 1e8
+counts); matrix, sinogram and image are resident in HBM before the timed
+region starts.
+
+Prints ONE JSON line with the metric, the per-kernel roofline of the dominant
+kernel (HIP events around every launch inside the timed region) and the CPU
+baseline (the repository's fp32 sequential restatement of the reference
+driver, timed on this host on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "MLEM iter/s + HBM GB/s (~200M nnz fp32); batched 64-frame vs plain speedup"
+
+
+def kernel_bytes(rows: int, cols: int, nnz: int) -> dict:
+    """Algorithmic HBM bytes per launch: every matrix record, index and vector
+    element counted once (u32 row/column pointers)."""
+    fwd = 8 * nnz + 4 * (rows + 1) + 4 * cols + 4 * rows + 4 * rows
+    bwd = 8 * nnz + 4 * (cols + 1) + 4 * rows + 4 * cols + 8 * cols
+    return {"fwd": fwd, "bwd": bwd, "iteration": fwd + bwd}
+
+
+def cpu_baseline(problem, iters: int, parity_x: np.ndarray | None):
+    from oracle import oracle as orc  # CPU baseline leg only
+    m = problem.matrix
+    x, _, _, _, secs = orc.mlem(m.rowidx, m.elements, m.cols, problem.y, iters)
+    med = float(np.median(secs))
+    out = {"value": 1.0 / med, "unit": "iterations/s", "cores": 1, "kind": "port",
+           "sample": f"{iters} iterations of the fp32 sequential restatement of seqcsr4mlem "
+                     f"(oracle/mlem_oracle.c) on the full benchmark matrix; median per-iteration "
+                     f"time {med:.3f} s"}
+    parity = None
+    if parity_x is not None:
+        d = np.abs(parity_x.astype(np.float64) - x.astype(np.float64))
+        floor = 1e-6 * float(np.max(np.abs(x)))
+        parity = {"iterations": iters,
+                  "rel_linf_vs_seq_fp32": float(np.max(d / np.maximum(np.abs(x), floor))),
+                  "rel_l2_vs_seq_fp32": float(np.linalg.norm(d) / np.linalg.norm(x))}
+    return out, parity
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="pet")
+    ap.add_argument("--mode", default="fast", choices=["fast", "atomic", "exact"])
+    ap.add_argument("--cpu-iters", type=int, default=4,
+                    help="iterations of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--launch", default="", help="fwd_lanes,fwd_bpc,bwd_lanes,bwd_bpc")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU row sharding is a design note, not built)", file=sys.stderr)
+        return 2
+
+    import mlem_amd as M
+    from mlem_amd import synth
+
+    t0 = time.time()
+    prob = synth.make_problem(args.workload)
+    m = prob.matrix
+    print(f"[bench] workload {args.workload}: {m.rows} rows, {m.cols} cols, {m.nnz} nnz "
+          f"(generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    t0 = time.time()
+    A = M.Csr4Matrix.from_arrays(m.rows, m.cols, m.rowidx, m.elements,
+                                 no_csc=(args.mode == "atomic"))
+    print(f"[bench] matrix in HBM + CSC in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    if args.launch:
+        A.set_launch(*[int(v) for v in args.launch.split(",")])
+    if args.autotune:
+        A.autotune()
+    launch = A.launch()
+
+    R = M.Recon(A, prob.y, mode=args.mode, use_graph=False)
+    if args.warmup:
+        R.run(args.warmup)
+    M.device_sync()
+    t_start = time.perf_counter()
+    kf, kb, span = R.run_timed(args.steps)
+    M.device_sync()
+    wall = time.perf_counter() - t_start
+    done = R.iterations
+    ms_per_step = 1000.0 * wall / args.steps
+    value = args.steps / wall
+
+    B = kernel_bytes(m.rows, m.cols, m.nnz)
+    fwd_ms, bwd_ms = float(np.mean(kf)), float(np.mean(kb))
+    dom = "fwd" if fwd_ms >= bwd_ms else "bwd"
+    dom_ms = max(fwd_ms, bwd_ms)
+    achieved = B[dom] / (dom_ms * 1e-3) / 1e9
+    name = {"fwd": "k_fwd (CSR forward projection + fused ratio)",
+            "bwd": "k_bwd (CSC gather backprojection + fused update)"}[dom]
+    if args.mode == "atomic":
+        name = {"fwd": "k_fwd (CSR forward projection + fused ratio)",
+                "bwd": "k_scatter + k_update (atomic backprojection)"}[dom]
+
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_{args.workload}_{args.mode}.json"
+    if tfile.exists():
+        t = json.loads(tfile.read_text())
+        if t.get("launch") == launch:
+            traffic = t.get(dom)
+
+    # parity sample + CPU baseline (same iteration count)
+    parity = None
+    cpu = None
+    if args.cpu_iters > 0:
+        R2 = M.Recon(A, prob.y, mode=args.mode)
+        R2.run(args.cpu_iters)
+        xg = R2.image()
+        R2.close()
+        cpu, parity = cpu_baseline(prob, args.cpu_iters, xg)
+        cpu["threads_available"] = os.cpu_count()
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.workload}: plain fp32 MLEM, {'x'.join(map(str, m.grid))} grid, "
+                        f"{m.rows} lines of response, {m.nnz} nonzeros, {m.cols} voxels",
+            "rows": m.rows, "cols": m.cols, "nnz": m.nnz, "mode": args.mode,
+            "launch": launch, "iterations_run": done,
+        },
+        "hbm_gbs_iteration": B["iteration"] / (ms_per_step * 1e-3) / 1e9,
+        "kernel_ms": {"fwd": fwd_ms, "bwd": bwd_ms, "span_per_iter": span / args.steps},
+        "roofline": {
+            "kernel": name,
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes": B[dom],
+            "other_kernel_frac": (B["bwd" if dom == "fwd" else "fwd"] /
+                                  (min(fwd_ms, bwd_ms) * 1e-3) / 1e9) / PEAK_HBM_GBS,
+        },
+        "cpu_baseline": cpu,
+        "parity": parity,
+    }
+    print(json.dumps(line), flush=True)
+    R.close()
+    A.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

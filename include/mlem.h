@@ -1,0 +1,232 @@
+/*
+ * mlem.h — C ABI of the MI355X MLEM reconstruction engine (libmlem.so).
+ *
+ * The entry points below replace the plain-MLEM path of the reference's
+ * sequential driver (mlem_seq/seqcsr4mlem.cpp) and the csr4 matrix class it
+ * calls (helper_files_common/csr4matrix.{hpp,cpp}). Each declaration names the
+ * reference function or method whose role it takes over.
+ *
+ * Conventions
+ *   - C linkage, no exceptions cross the boundary.
+ *   - Every function returns an int status: MLEM_OK (0) or a negative code;
+ *     mlem_last_error() then returns a thread-local message.
+ *   - Host pointers are caller-allocated; sizes are passed explicitly and are
+ *     checked against the matrix dimensions.
+ *   - A matrix handle owns one HIP stream and its device buffers. A handle is
+ *     not thread-safe; use one handle per host thread.
+ *   - Matrix storage on the device: CSR {u32 column, f32 value} (the csr4
+ *     element layout, 8 B) plus a CSC transpose {u32 row, f32 value} built on
+ *     the device with rows kept in ascending order inside every column.
+ *
+ * Compute modes (mlem_mode):
+ *   MLEM_MODE_FAST   sub-wavefront row/column groups, fp32 fused multiply-add,
+ *                    tree reductions; deterministic run to run.
+ *   MLEM_MODE_EXACT  one lane per row (forward) and per column (backward),
+ *                    ascending element order, separate multiply and add,
+ *                    flush-to-zero: reproduces the reference's fp32 rounding
+ *                    sequence bit for bit.
+ *   MLEM_MODE_ATOMIC forward as FAST; backprojection by fp32 atomic scatter
+ *                    over the CSR (non-deterministic summation order).
+ */
+#ifndef MLEM_H
+#define MLEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Only the entry points below are exported from libmlem.so. */
+#define MLEM_API __attribute__((visibility("default")))
+
+/* ---- status codes ---------------------------------------------------- */
+#define MLEM_OK 0
+#define MLEM_EINVAL (-1)  /* bad argument or size mismatch */
+#define MLEM_EIO (-2)     /* file could not be opened / read / written */
+#define MLEM_EFORMAT (-3) /* not a csr4 v4 file, truncated, inconsistent */
+#define MLEM_ENOMEM (-4)  /* host or device allocation failed */
+#define MLEM_EDEVICE (-5) /* no usable GPU, or a HIP call failed */
+#define MLEM_ESTATE (-6)  /* call not valid in the handle's current state */
+
+/* ---- compute modes --------------------------------------------------- */
+#define MLEM_MODE_FAST 0
+#define MLEM_MODE_EXACT 1
+#define MLEM_MODE_ATOMIC 2
+
+/* ---- open flags ------------------------------------------------------ */
+#define MLEM_OPEN_NO_CSC 0x1u /* skip the CSC build (ATOMIC mode only)   */
+
+typedef struct mlem_matrix mlem_matrix; /* device-resident system matrix */
+typedef struct mlem_recon mlem_recon;   /* one reconstruction's state    */
+
+/* Thread-local message describing the last failing call ("" if none). */
+MLEM_API const char* mlem_last_error(void);
+
+/* Library version string, e.g. "mlem-amd 0.1 gfx950". */
+MLEM_API const char* mlem_version(void);
+
+/* ---- matrix ------------------------------------------------------------ */
+
+/* Open a csr4 v4 file, validate it, stream the element array into HBM once
+ * and build the CSC transpose on the device.
+ * Replaces Csr4Matrix::Csr4Matrix(const std::string&)
+ * (helper_files_common/csr4matrix.cpp:32-157) together with
+ * Csr4Matrix::mapRows(0, rows()) (csr4matrix.cpp:166-224). Errors carry the
+ * reference's messages ("Cannot open file ...", "... is not a CSR matrix",
+ * "... has wrong CSR version (need v4)", "... is too short ...",
+ * "Cannot find scanner geometry configuration",
+ * "Error reading scanner geometry configuration"). Unlike the reference the
+ * size check is 64-bit and row offsets and column indices are validated. */
+MLEM_API int mlem_open(const char* path, unsigned flags, mlem_matrix** out);
+
+/* Parse and validate a csr4 file's header, scanner block and row index
+ * without touching a GPU: the header half of Csr4Matrix::Csr4Matrix
+ * (csr4matrix.cpp:32-147), same error messages as mlem_open. grid and
+ * symflags may be NULL. */
+MLEM_API int mlem_probe(const char* path, uint32_t* rows, uint32_t* cols,
+                        uint64_t* nnz, uint32_t grid[3], uint32_t* symflags);
+
+/* Same as mlem_open for a matrix already in host memory: rowidx holds the
+ * csr4 row index (nRows u64 exclusive END offsets, csr4matrix.cpp:141-147,
+ * csr4matrix.hpp:138-149) and elements holds nnz 8-byte {u32 column,
+ * f32 value} records (helper_files_common/matrixelement.hpp:37-46).
+ * Counterpart of constructing a Csr4Matrix and installing the element array
+ * with Csr4Matrix::setData (csr4matrix.hpp:46). */
+MLEM_API int mlem_open_host(uint32_t rows, uint32_t cols, uint64_t nnz,
+                   const uint64_t* rowidx, const void* elements,
+                   unsigned flags, mlem_matrix** out);
+
+/* Release device memory, stream and handle. Replaces Csr4Matrix::~Csr4Matrix
+ * (csr4matrix.cpp:159-163). Accepts NULL. */
+MLEM_API void mlem_close(mlem_matrix* m);
+
+/* rows(), columns(), elements() (csr4matrix.hpp:35-37). */
+MLEM_API int mlem_dims(const mlem_matrix* m, uint32_t* rows, uint32_t* cols,
+              uint64_t* nnz);
+
+/* Grid dimensions and symmetry flags from the csr4 scanner block
+ * (csr4matrix.cpp:62-65,126-129; Csr4Matrix::scannerConfig/symconfig,
+ * csr4matrix.hpp:39-40). Zeros for matrices opened with mlem_open_host. */
+MLEM_API int mlem_geometry(const mlem_matrix* m, uint32_t grid[3], uint32_t* symflags);
+
+/* The handle's HIP stream (a hipStream_t), for callers that order their own
+ * work against the library's. */
+MLEM_API void* mlem_stream(mlem_matrix* m);
+
+/* ---- step functions on host buffers (kernel-level parity) --------------- */
+
+/* norm_j = sum_i a_ij in ascending row order, fp32.
+ * Replaces calcColumnSums (mlem_seq/seqcsr4mlem.cpp:37-49). */
+MLEM_API int mlem_column_sums(mlem_matrix* m, float* norm, uint32_t cols);
+
+/* x_j = (float)(sum(y) / sum(norm)) with both sums in double, index order.
+ * Replaces initImage (seqcsr4mlem.cpp:52-66). initial may be NULL. */
+MLEM_API int mlem_init_image(mlem_matrix* m, const float* norm, uint32_t cols,
+                    const int32_t* y, uint32_t rows, float* x, float* initial);
+
+/* f = A x. Replaces calcFwProj (seqcsr4mlem.cpp:69-91). */
+MLEM_API int mlem_fwproj(mlem_matrix* m, const float* x, uint32_t cols, float* f,
+                uint32_t rows, int mode);
+
+/* r_i = (f_i != 0) ? (float)y_i / f_i : 0.
+ * Replaces calcCorrel (seqcsr4mlem.cpp:94-105). */
+MLEM_API int mlem_correl(mlem_matrix* m, const float* f, const int32_t* y, float* r,
+                uint32_t rows);
+
+/* u = A^T r. Replaces calcBkProj (seqcsr4mlem.cpp:108-120). */
+MLEM_API int mlem_bkproj(mlem_matrix* m, const float* r, uint32_t rows, float* u,
+                uint32_t cols, int mode);
+
+/* x_j *= (n_j != 0) ? u_j / n_j : u_j.
+ * Replaces calcUpdate (seqcsr4mlem.cpp:123-134). */
+MLEM_API int mlem_update(mlem_matrix* m, const float* u, const float* norm, float* x,
+                uint32_t cols);
+
+/* ---- whole reconstruction ------------------------------------------------ */
+
+#define MLEM_INIT_REFERENCE 0 /* x0 = (float)(sum y / sum norm), initImage */
+#define MLEM_INIT_GIVEN 1     /* x0 = caller's x_inout                     */
+
+typedef struct mlem_options {
+  int mode;               /* MLEM_MODE_*                                  */
+  int init;               /* MLEM_INIT_*                                  */
+  int checkpoint_interval;/* >0: write img.chkpt/iter.chkpt every N iters */
+  const char* checkpoint_dir; /* directory of the checkpoint files (NULL = CWD) */
+  int verbose;            /* 1: print the reference driver's stdout lines */
+  int use_graph;          /* 1: run iterations from a captured hipGraph   */
+  double* loglik;         /* optional [iterations]: FP64 Poisson log-likelihood
+                             of each iterate's forward projection          */
+  double* fwd_total;      /* optional [iterations]: FP64 sum_i f_i         */
+} mlem_options;
+
+/* Fill *o with the defaults (FAST, reference init, no checkpoints, quiet,
+ * graph capture on). */
+MLEM_API void mlem_options_default(mlem_options* o);
+
+/* Run `iterations` MLEM iterations for one int32 sinogram. x_inout receives
+ * the image (and supplies x0 under MLEM_INIT_GIVEN). With verbose=1 the
+ * reference's stdout lines are printed (INIT line, per-iteration image sum,
+ * zero and subnormal counts). Replaces mlem() (seqcsr4mlem.cpp:137-189); the
+ * checkpoint files and restart follow mpicsr4mlem.cpp:95-154,536-553 with the
+ * interval meaning of hybCalcFunc.cu:852-853. */
+MLEM_API int mlem_reconstruct(mlem_matrix* m, const int32_t* y, uint32_t rows,
+                     float* x_inout, uint32_t cols, int iterations,
+                     const mlem_options* opts);
+
+/* ---- device-resident reconstruction state --------------------------------
+ * The same iteration as mlem_reconstruct, split so that the sinogram and the
+ * image stay in HBM between calls (used by the benchmark and the batched
+ * driver). No reference counterpart beyond the loop body of mlem()
+ * (seqcsr4mlem.cpp:163-188). */
+
+/* Upload y, compute x0 (opts->init), allocate r. */
+MLEM_API int mlem_recon_create(mlem_matrix* m, const int32_t* y, uint32_t rows,
+                      const float* x0, uint32_t cols, const mlem_options* opts,
+                      mlem_recon** out);
+MLEM_API void mlem_recon_destroy(mlem_recon* r);
+/* Enqueue `iterations` iterations on the matrix stream and wait for them. */
+MLEM_API int mlem_recon_run(mlem_recon* r, int iterations);
+/* As mlem_recon_run, with HIP events around every kernel launch; returns the
+ * per-kernel durations in milliseconds (arrays of length `iterations`, any
+ * may be NULL) and the whole span. kernel_fwd: forward+ratio launches;
+ * kernel_bwd: backprojection+update launches (ATOMIC mode: scatter and
+ * update summed). */
+MLEM_API int mlem_recon_run_timed(mlem_recon* r, int iterations, float* kernel_fwd_ms,
+                         float* kernel_bwd_ms, float* span_ms);
+/* Copy the current image to the host. */
+MLEM_API int mlem_recon_image(mlem_recon* r, float* x, uint32_t cols);
+/* Iterations completed so far. */
+MLEM_API int mlem_recon_iterations(const mlem_recon* r);
+
+/* ---- launch configuration (autotuner) --------------------------------------
+ * lanes: lanes per row (forward) / per column (backward) in FAST mode, one of
+ * 4, 8, 16, 32, 64; blocks_per_cu: 0 = one 256-lane block per 256/lanes
+ * rows, else a grid of blocks_per_cu * CUs blocks walking contiguous chunks.
+ * Defaults come from the average row/column length. No reference
+ * counterpart (the reference hard-codes 1024-thread blocks,
+ * mlem_cuda/versions/version_5.cu:121-126). */
+MLEM_API int mlem_get_launch(const mlem_matrix* m, int* fwd_lanes,
+                             int* fwd_blocks_per_cu, int* bwd_lanes,
+                             int* bwd_blocks_per_cu);
+MLEM_API int mlem_set_launch(mlem_matrix* m, int fwd_lanes,
+                             int fwd_blocks_per_cu, int bwd_lanes,
+                             int bwd_blocks_per_cu);
+/* Time every forward and backward configuration with HIP events on this
+ * matrix and keep the fastest of each. */
+MLEM_API int mlem_autotune(mlem_matrix* m);
+
+/* ---- measurement helper ---------------------------------------------------
+ * Copy `bytes` (multiple of 16) from one device buffer to another with a plain
+ * streaming kernel, `repeats` times; average kernel time in *ms. Used to
+ * calibrate hardware byte counters. No reference counterpart. */
+MLEM_API int mlem_calibration_copy(size_t bytes, int repeats, float* ms);
+
+/* Synchronize the device (hipDeviceSynchronize). */
+MLEM_API int mlem_device_sync(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MLEM_H */

@@ -1,0 +1,295 @@
+"""MI355X-native MLEM reconstruction engine — Python face of libmlem.so.
+
+The names mirror the reference's sequential driver and matrix class so that
+tests read like that code:
+
+  Csr4Matrix            helper_files_common/csr4matrix.hpp:30-40 (here: the
+                        matrix lives in HBM, CSR + device-built CSC)
+  calcColumnSums        mlem_seq/seqcsr4mlem.cpp:37-49
+  initImage             seqcsr4mlem.cpp:52-66
+  calcFwProj            seqcsr4mlem.cpp:69-91
+  calcCorrel            seqcsr4mlem.cpp:94-105
+  calcBkProj            seqcsr4mlem.cpp:108-120
+  calcUpdate            seqcsr4mlem.cpp:123-134
+  mlem                  seqcsr4mlem.cpp:137-189
+
+Every call goes through the C ABI into HIP kernels; errors raise MlemError
+(the reference throws std::runtime_error). There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from ._native import (INIT_GIVEN, INIT_REFERENCE, MODE_ATOMIC, MODE_EXACT, MODE_FAST, MODES,
+                      OPEN_NO_CSC, MlemError, Options, check, lib)
+
+__all__ = [
+    "Csr4Matrix", "Recon", "MlemError", "ELEMENT_DTYPE", "probe", "calcColumnSums", "initImage",
+    "calcFwProj", "calcCorrel", "calcBkProj", "calcUpdate", "mlem", "MODE_FAST", "MODE_EXACT",
+    "MODE_ATOMIC", "calibration_copy", "device_sync",
+]
+
+# csr4 element record {u32 column, f32 value} (matrixelement.hpp:37-46)
+ELEMENT_DTYPE = np.dtype([("column", "<u4"), ("value", "<f4")])
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _mode(mode) -> int:
+    if isinstance(mode, str):
+        return MODES[mode]
+    return int(mode)
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def probe(path: str) -> dict:
+    """Header of a csr4 file (no GPU needed)."""
+    rows, cols, grid, sym = C.c_uint32(), C.c_uint32(), (C.c_uint32 * 3)(), C.c_uint32()
+    nnz = C.c_uint64()
+    check(lib().mlem_probe(os.fsencode(path), C.byref(rows), C.byref(cols), C.byref(nnz), grid,
+                           C.byref(sym)))
+    return {"rows": rows.value, "cols": cols.value, "nnz": nnz.value, "grid": tuple(grid),
+            "symflags": sym.value}
+
+
+class Csr4Matrix:
+    """Device-resident csr4 system matrix (Csr4Matrix, csr4matrix.hpp:30-40).
+
+    Csr4Matrix(path)                            load a csr4 file into HBM
+    Csr4Matrix.from_arrays(rows, cols, rowidx, elements)   from host memory
+    """
+
+    def __init__(self, path: Optional[str] = None, *, no_csc: bool = False, _handle=None):
+        self._h = None
+        if _handle is not None:
+            self._h = _handle
+        else:
+            if path is None:
+                raise ValueError("path required")
+            h = C.c_void_p()
+            check(lib().mlem_open(os.fsencode(path), OPEN_NO_CSC if no_csc else 0, C.byref(h)))
+            self._h = h
+        r, c, n = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        check(lib().mlem_dims(self._h, C.byref(r), C.byref(c), C.byref(n)))
+        self._rows, self._cols, self._nnz = r.value, c.value, n.value
+
+    @classmethod
+    def from_arrays(cls, rows: int, cols: int, rowidx: np.ndarray, elements: np.ndarray,
+                    no_csc: bool = False) -> "Csr4Matrix":
+        rowidx = np.ascontiguousarray(rowidx, dtype=np.uint64)
+        if elements.dtype != ELEMENT_DTYPE:
+            elements = np.ascontiguousarray(elements).view(ELEMENT_DTYPE)
+        elements = np.ascontiguousarray(elements)
+        if rowidx.shape != (rows,):
+            raise ValueError("rowidx must hold one END offset per row")
+        nnz = int(elements.shape[0])
+        h = C.c_void_p()
+        check(lib().mlem_open_host(rows, cols, nnz, _ptr(rowidx), _ptr(elements),
+                                   OPEN_NO_CSC if no_csc else 0, C.byref(h)))
+        return cls(_handle=h)
+
+    def rows(self) -> int:
+        return self._rows
+
+    def columns(self) -> int:
+        return self._cols
+
+    def elements(self) -> int:
+        return self._nnz
+
+    def geometry(self) -> tuple:
+        grid, sym = (C.c_uint32 * 3)(), C.c_uint32()
+        check(lib().mlem_geometry(self._h, grid, C.byref(sym)))
+        return tuple(grid), sym.value
+
+    def launch(self) -> dict:
+        v = [C.c_int() for _ in range(4)]
+        check(lib().mlem_get_launch(self.handle, *[C.byref(a) for a in v]))
+        return {"fwd_lanes": v[0].value, "fwd_blocks_per_cu": v[1].value,
+                "bwd_lanes": v[2].value, "bwd_blocks_per_cu": v[3].value}
+
+    def set_launch(self, fwd_lanes, fwd_blocks_per_cu, bwd_lanes, bwd_blocks_per_cu) -> None:
+        check(lib().mlem_set_launch(self.handle, fwd_lanes, fwd_blocks_per_cu, bwd_lanes,
+                                    bwd_blocks_per_cu))
+
+    def autotune(self) -> dict:
+        check(lib().mlem_autotune(self.handle))
+        return self.launch()
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise MlemError(-6, "matrix is closed")
+        return self._h
+
+    def close(self) -> None:
+        if self._h is not None:
+            lib().mlem_close(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def calcColumnSums(matrix: Csr4Matrix) -> np.ndarray:
+    norm = np.empty(matrix.columns(), np.float32)
+    check(lib().mlem_column_sums(matrix.handle, _ptr(norm), norm.size))
+    return norm
+
+
+def initImage(matrix: Csr4Matrix, norm, lmsino) -> tuple[np.ndarray, float]:
+    norm, y = _f32(norm), _i32(lmsino)
+    x = np.empty(matrix.columns(), np.float32)
+    init = C.c_float()
+    check(lib().mlem_init_image(matrix.handle, _ptr(norm), norm.size, _ptr(y), y.size, _ptr(x),
+                                C.byref(init)))
+    return x, init.value
+
+
+def calcFwProj(matrix: Csr4Matrix, image, mode="fast") -> np.ndarray:
+    x = _f32(image)
+    f = np.empty(matrix.rows(), np.float32)
+    check(lib().mlem_fwproj(matrix.handle, _ptr(x), x.size, _ptr(f), f.size, _mode(mode)))
+    return f
+
+
+def calcCorrel(matrix: Csr4Matrix, fwproj, lmsino) -> np.ndarray:
+    f, y = _f32(fwproj), _i32(lmsino)
+    if f.size != y.size:
+        raise MlemError(-1, "fwproj and sinogram sizes differ")
+    r = np.empty(f.size, np.float32)
+    check(lib().mlem_correl(matrix.handle, _ptr(f), _ptr(y), _ptr(r), r.size))
+    return r
+
+
+def calcBkProj(matrix: Csr4Matrix, correlation, mode="fast") -> np.ndarray:
+    r = _f32(correlation)
+    u = np.empty(matrix.columns(), np.float32)
+    check(lib().mlem_bkproj(matrix.handle, _ptr(r), r.size, _ptr(u), u.size, _mode(mode)))
+    return u
+
+
+def calcUpdate(matrix: Csr4Matrix, update, norm, image: np.ndarray) -> np.ndarray:
+    """x *= (n != 0) ? u/n : u, in place on `image` (float32, contiguous)."""
+    u, n = _f32(update), _f32(norm)
+    if image.dtype != np.float32 or not image.flags.c_contiguous:
+        raise ValueError("image must be a contiguous float32 array")
+    check(lib().mlem_update(matrix.handle, _ptr(u), _ptr(n), _ptr(image), image.size))
+    return image
+
+
+def _options(mode="fast", x0_given=False, checkpoint_interval=0, checkpoint_dir=None,
+             verbose=False, use_graph=True, loglik=None, fwd_total=None) -> Options:
+    o = Options()
+    lib().mlem_options_default(C.byref(o))
+    o.mode = _mode(mode)
+    o.init = INIT_GIVEN if x0_given else INIT_REFERENCE
+    o.checkpoint_interval = int(checkpoint_interval)
+    o.checkpoint_dir = os.fsencode(checkpoint_dir) if checkpoint_dir else None
+    o.verbose = 1 if verbose else 0
+    o.use_graph = 1 if use_graph else 0
+    if loglik is not None:
+        o.loglik = loglik.ctypes.data_as(C.POINTER(C.c_double))
+    if fwd_total is not None:
+        o.fwd_total = fwd_total.ctypes.data_as(C.POINTER(C.c_double))
+    return o
+
+
+def mlem(matrix: Csr4Matrix, lmsino, image: Optional[np.ndarray], nIterations: int, *,
+         mode="fast", x0_given=False, checkpoint_interval=0, checkpoint_dir=None,
+         verbose=False, use_graph=True, stats=False):
+    """Reconstruct; returns the image (and (loglik, fwd_total) when stats).
+
+    image: float32[cols], receives the result; holds x0 when x0_given.
+    """
+    y = _i32(lmsino)
+    if image is None:
+        image = np.zeros(matrix.columns(), np.float32)
+    if image.dtype != np.float32 or not image.flags.c_contiguous:
+        raise ValueError("image must be a contiguous float32 array")
+    ll = np.zeros(max(nIterations, 1), np.float64) if stats else None
+    ft = np.zeros(max(nIterations, 1), np.float64) if stats else None
+    o = _options(mode, x0_given, checkpoint_interval, checkpoint_dir, verbose, use_graph, ll, ft)
+    check(lib().mlem_reconstruct(matrix.handle, _ptr(y), y.size, _ptr(image), image.size,
+                                 int(nIterations), C.byref(o)))
+    if stats:
+        return image, ll[:nIterations], ft[:nIterations]
+    return image
+
+
+class Recon:
+    """Device-resident reconstruction state (sinogram, image, ratio in HBM)."""
+
+    def __init__(self, matrix: Csr4Matrix, lmsino, x0=None, *, mode="fast", use_graph=True):
+        self.matrix = matrix
+        y = _i32(lmsino)
+        x = None if x0 is None else _f32(x0)
+        o = _options(mode, x is not None, use_graph=use_graph)
+        h = C.c_void_p()
+        check(lib().mlem_recon_create(matrix.handle, _ptr(y), y.size, _ptr(x),
+                                      matrix.columns(), C.byref(o), C.byref(h)))
+        self._h = h
+
+    def run(self, iterations: int) -> None:
+        check(lib().mlem_recon_run(self._h, int(iterations)))
+
+    def run_timed(self, iterations: int):
+        """Returns (fwd_ms[iters], bwd_ms[iters], span_ms) from HIP events."""
+        kf = np.zeros(max(iterations, 1), np.float32)
+        kb = np.zeros(max(iterations, 1), np.float32)
+        span = C.c_float()
+        check(lib().mlem_recon_run_timed(self._h, int(iterations), _ptr(kf), _ptr(kb),
+                                         C.byref(span)))
+        return kf[:iterations], kb[:iterations], span.value
+
+    def image(self) -> np.ndarray:
+        x = np.empty(self.matrix.columns(), np.float32)
+        check(lib().mlem_recon_image(self._h, _ptr(x), x.size))
+        return x
+
+    @property
+    def iterations(self) -> int:
+        return lib().mlem_recon_iterations(self._h)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            lib().mlem_recon_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def calibration_copy(nbytes: int, repeats: int = 5) -> float:
+    ms = C.c_float()
+    check(lib().mlem_calibration_copy(int(nbytes), int(repeats), C.byref(ms)))
+    return ms.value
+
+
+def device_sync() -> None:
+    check(lib().mlem_device_sync())

@@ -1,0 +1,98 @@
+// internal.hpp — device-side matrix/state structures and kernel launchers
+// shared by the C ABI (api.cpp) and the kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+namespace mlem {
+
+// Error raised inside the library; the C ABI converts it to a status code.
+struct ApiError {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void raise(int code, const std::string& msg);
+void hip_check(hipError_t e, const char* what);
+#define MLEM_HIP(call) ::mlem::hip_check((call), #call)
+
+// Launch geometry of a row- or column-group kernel.
+struct GroupLaunch {
+  int lanes = 16;          // lanes per row (or column); 1 = exact sequential
+  int blocks_per_cu = 0;   // 0: one block per 256/lanes rows, no chunk loop
+};
+
+struct DevMatrix {
+  uint32_t rows = 0, cols = 0;
+  uint64_t nnz = 0;
+  uint32_t grid[3] = {0, 0, 0};
+  uint32_t symflags = 0;
+  int device = 0;
+  int cus = 256;
+  hipStream_t stream = nullptr;
+
+  uint32_t* rowptr = nullptr;  // [rows+1]
+  uint2* csr = nullptr;        // [nnz] {column, value bits}
+  uint32_t* colptr = nullptr;  // [cols+1]
+  uint2* csc = nullptr;        // [nnz] {row, value bits}, rows ascending
+  float* norm = nullptr;       // [cols] fp32 column sums, ascending row order
+  double sumnorm = 0.0;        // sequential double sum of norm (initImage)
+
+  GroupLaunch fwd, bwd, scatter;
+
+  // per-handle scratch for the host-buffer step functions
+  float* sx = nullptr;   // [cols]
+  float* su = nullptr;   // [cols]
+  float* sf = nullptr;   // [rows]
+  float* sr = nullptr;   // [rows]
+  int32_t* sy = nullptr; // [rows]
+};
+
+// Deterministic FP64 reduction state for the forward pass: per-block partial
+// (sum of y log f - f, sum of f) pairs, a completion counter, and a ring of
+// per-iteration results indexed by a device-side iteration counter.
+struct StatsState {
+  double2* partials = nullptr;  // [max blocks]
+  unsigned* counter = nullptr;  // [2]: blocks done, next result slot
+  double2* results = nullptr;   // [ring]
+  unsigned ring = 0;
+  unsigned max_blocks = 0;
+};
+
+// ---- launchers (kernels.hip) -------------------------------------------
+// Grid size of a statistics-carrying forward launch (partials needed).
+unsigned stats_max_blocks(const DevMatrix& m);
+// Forward projection. ratio: out = (f != 0) ? y/f : 0, else out = f.
+void launch_fwd(const DevMatrix& m, int mode, const float* x, const int32_t* y,
+                float* out, bool ratio, const StatsState* stats, hipStream_t s);
+// Backprojection through the CSC. update: x *= (n != 0) ? u/n : u written to
+// `out` (which holds x), else out = u.
+void launch_bwd(const DevMatrix& m, int mode, const float* r, const float* norm,
+                float* out, bool update, hipStream_t s);
+// Backprojection by atomic scatter over the CSR: u += A^T r (u pre-zeroed).
+void launch_scatter(const DevMatrix& m, const float* r, float* u, hipStream_t s);
+// x *= (n != 0) ? u/n : u; optionally u = 0 afterwards.
+void launch_update(uint32_t cols, const float* u, const float* norm, float* x,
+                   bool zero_u, hipStream_t s);
+// r = (f != 0) ? y/f : 0.
+void launch_correl(uint32_t rows, const float* f, const int32_t* y, float* r,
+                   hipStream_t s);
+// norm from the CSC, sequential per column in ascending row order.
+void launch_colsums_csc(const DevMatrix& m, float* norm, hipStream_t s);
+// norm by atomic scatter (matrices opened without a CSC).
+void launch_colsums_atomic(const DevMatrix& m, float* norm, hipStream_t s);
+// Count column indices >= cols in the CSR (validation).
+void launch_check_columns(const DevMatrix& m, unsigned* bad, hipStream_t s);
+// Build rowptr (u32) from the csr4 END-offset row index already on device.
+void launch_rowptr_from_rowidx(const uint64_t* rowidx, uint32_t rows,
+                               uint32_t* rowptr, hipStream_t s);
+// Build colptr/csc from the CSR (stable: rows ascending in every column).
+void build_csc(DevMatrix& m, hipStream_t s);
+// Fill a float buffer.
+void launch_fill(float* p, float v, size_t n, hipStream_t s);
+// Streaming copy kernel (byte-counter calibration).
+void launch_copy(const uint4* src, uint4* dst, size_t n16, hipStream_t s);
+
+}  // namespace mlem

@@ -1,0 +1,313 @@
+"""HIP path against the oracle on the GPU (through the C ABI).
+
+EXACT mode must reproduce the fp32 restatement of seqcsr4mlem bit for bit;
+FAST and ATOMIC modes are held to stated relative tolerances against the fp32
+restatement and the FP64 oracle. Sizes are those the oracle finishes in
+seconds; the 200M-element workload is covered by size-independent
+properties (adjointness, count preservation) in test_gpu_scale.py.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, rel_linf
+import mlem_amd as M
+from mlem_amd import synth
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+# Relative L-infinity tolerances (floor: 1e-6 of the image maximum) of the
+# FAST/ATOMIC GPU path: single projections vs FP64, and the 60-iteration
+# reconstruction vs the fp32 sequential restatement.
+TOL_PROJ = 2e-6
+TOL_RECON = 1e-4
+
+
+def _open(m, **kw):
+    return M.Csr4Matrix.from_arrays(m.rows, m.cols, m.rowidx, m.elements, **kw)
+
+
+@pytest.fixture(scope="module")
+def tiny(tiny_problem):
+    A = _open(tiny_problem.matrix)
+    yield tiny_problem, A
+    A.close()
+
+
+@pytest.fixture(scope="module")
+def small():
+    p = synth.make_problem("small")
+    A = _open(p.matrix)
+    yield p, A
+    A.close()
+
+
+def test_golden_fixture_exact(golden_tiny):
+    ref, rowidx, el = golden_tiny
+    A = M.Csr4Matrix(str(GOLDEN / "tiny3x4.csr4"))
+    assert (A.rows(), A.columns(), A.elements()) == (3, 4, 6)
+    assert A.geometry() == ((2, 2, 1), 0)
+    y = np.array(ref["sino"], np.int32)
+    x = M.mlem(A, y, None, 3, mode="exact")
+    xo = orc.mlem(rowidx, el, 4, y, 3)[0]
+    assert np.array_equal(x, xo)
+    assert rel_linf(x, ref["iterates"][-1]["x"]) < 2e-6
+
+
+def test_column_sums_exact(tiny):
+    p, A = tiny
+    m = p.matrix
+    assert np.array_equal(M.calcColumnSums(A), orc.column_sums(m.rowidx, m.elements, m.cols))
+
+
+def test_steps_exact(tiny):
+    p, A = tiny
+    m = p.matrix
+    rng = np.random.default_rng(3)
+    x = rng.uniform(0.1, 3, m.cols).astype(np.float32)
+    f = M.calcFwProj(A, x, "exact")
+    assert np.array_equal(f, orc.fwproj(m.rowidx, m.elements, x))
+    r = M.calcCorrel(A, f, p.y)
+    assert np.array_equal(r, orc.correl(f, p.y))
+    u = M.calcBkProj(A, r, "exact")
+    assert np.array_equal(u, orc.bkproj(m.rowidx, m.elements, r, m.cols))
+    norm = M.calcColumnSums(A)
+    xg = M.calcUpdate(A, u, norm, x.copy())
+    assert np.array_equal(xg, orc.update(u, norm, x))
+    x0, v = M.initImage(A, norm, p.y)
+    xo, vo, _, _ = orc.init_image(norm, p.y)
+    assert v == vo and np.array_equal(x0, xo)
+
+
+def test_steps_fast_and_atomic(tiny):
+    p, A = tiny
+    m = p.matrix
+    rng = np.random.default_rng(4)
+    x = rng.uniform(0.1, 3, m.cols).astype(np.float32)
+    r = rng.uniform(0.1, 3, m.rows).astype(np.float32)
+    f64 = orc.f64_fwproj(m.rowidx, m.elements, x)
+    u64 = orc.f64_bkproj(m.rowidx, m.elements, r, m.cols)
+    assert rel_linf(M.calcFwProj(A, x, "fast"), f64) < TOL_PROJ
+    assert rel_linf(M.calcBkProj(A, r, "fast"), u64) < TOL_PROJ
+    assert rel_linf(M.calcBkProj(A, r, "atomic"), u64) < TOL_PROJ
+
+
+def test_recon_exact_bitwise_small(small):
+    # the 32^3, 262,144-line workload: 60 iterations, reference init
+    p, A = small
+    m = p.matrix
+    x = M.mlem(A, p.y, None, 60, mode="exact")
+    xo = orc.mlem(m.rowidx, m.elements, m.cols, p.y, 60)[0]
+    assert np.array_equal(x, xo)
+
+
+@pytest.mark.parametrize("mode", ["fast", "atomic"])
+def test_recon_fast_modes_small(small, mode):
+    p, A = small
+    m = p.matrix
+    x = M.mlem(A, p.y, None, 60, mode=mode)
+    xo = orc.mlem(m.rowidx, m.elements, m.cols, p.y, 60)[0]
+    x64, _, _ = orc.f64_mlem(m.rowidx, m.elements, m.cols, p.y.astype(np.float64), 60)
+    e32 = rel_linf(x, xo)
+    e64 = rel_linf(x, x64)
+    e_ref64 = rel_linf(xo, x64)
+    print(f"{mode}: rel Linf vs fp32 seq {e32:.3e}, vs fp64 {e64:.3e}; seq vs fp64 {e_ref64:.3e}")
+    assert e32 < TOL_RECON
+    assert e64 < TOL_RECON
+
+
+def test_fast_is_deterministic_and_graph_matches(tiny):
+    p, A = tiny
+    a = M.mlem(A, p.y, None, 10, mode="fast", use_graph=True)
+    b = M.mlem(A, p.y, None, 10, mode="fast", use_graph=False)
+    c = M.mlem(A, p.y, None, 10, mode="fast", use_graph=True)
+    assert np.array_equal(a, b) and np.array_equal(a, c)
+
+
+def test_recon_state_matches_one_shot(tiny):
+    p, A = tiny
+    full = M.mlem(A, p.y, None, 7, mode="fast")
+    R = M.Recon(A, p.y, mode="fast")
+    R.run(3)
+    R.run(4)
+    assert R.iterations == 7
+    assert np.array_equal(R.image(), full)
+    R2 = M.Recon(A, p.y, mode="fast")
+    kf, kb, span = R2.run_timed(7)
+    assert np.all(kf > 0) and np.all(kb > 0) and span >= kf.sum()
+    assert np.array_equal(R2.image(), full)
+
+
+def test_stats_loglik_and_counts(tiny):
+    p, A = tiny
+    m = p.matrix
+    K = 12
+    x, ll, ft = M.mlem(A, p.y, None, K, mode="fast", stats=True)
+    x64, ll64, ft64 = orc.f64_mlem(m.rowidx, m.elements, m.cols, p.y.astype(np.float64), K)
+    assert rel_linf(ll, ll64) < 1e-6
+    assert rel_linf(ft, ft64) < 1e-6
+    assert np.all(np.diff(ll) > -1e-6 * np.abs(ll[:-1]))
+    # stats do not change the image
+    assert np.array_equal(x, M.mlem(A, p.y, None, K, mode="fast"))
+
+
+def test_given_x0_and_homogeneity(tiny):
+    p, A = tiny
+    x0 = np.full(A.columns(), 0.25, np.float32)
+    xa = M.mlem(A, p.y, x0.copy(), 10, mode="exact", x0_given=True)
+    xo = orc.mlem(p.matrix.rowidx, p.matrix.elements, A.columns(), p.y, 10, x0=x0)[0]
+    assert np.array_equal(xa, xo)
+    xb = M.mlem(A, p.y, (4 * x0).copy(), 10, mode="fast", x0_given=True)
+    assert rel_linf(xb, xa) < TOL_RECON
+
+
+def test_edge_cases_empty_rows_and_columns():
+    # rows with no elements, an empty column, one long row, zero counts
+    rows = [[], [(0, 1.0), (2, 0.5)], [], [(c, 0.25) for c in range(0, 300, 2) if c != 4],
+            [(1, 2.0)], []]
+    ends = np.cumsum([len(r) for r in rows]).astype(np.uint64)
+    el = np.array([e for r in rows for e in r], dtype=M.ELEMENT_DTYPE)
+    y = np.int32([0, 5, 3, 40, 0, 9])
+    A = M.Csr4Matrix.from_arrays(6, 301, ends, el)
+    for mode in ("exact", "fast", "atomic"):
+        x = M.mlem(A, y, None, 5, mode=mode)
+        xo = orc.mlem(ends, el, 301, y, 5)[0]
+        if mode == "exact":
+            assert np.array_equal(x, xo)
+        else:
+            assert rel_linf(x, xo) < 1e-5
+        assert x[299] == 0 and x[300] == 0 and x[4] == 0   # empty columns
+    A.close()
+
+
+def test_ragged_row_lengths():
+    # lengths 1..2000 exercise every lane-group tail
+    rng = np.random.default_rng(8)
+    cols = 5000
+    rows = []
+    for L in list(range(1, 300)) + [511, 512, 513, 1000, 2000]:
+        c = np.sort(rng.choice(cols, L, replace=False))
+        rows.append([(int(j), float(np.float32(rng.uniform(0.1, 1.7)))) for j in c])
+    ends = np.cumsum([len(r) for r in rows]).astype(np.uint64)
+    el = np.array([e for r in rows for e in r], dtype=M.ELEMENT_DTYPE)
+    A = M.Csr4Matrix.from_arrays(len(rows), cols, ends, el)
+    x = rng.uniform(0.1, 3, cols).astype(np.float32)
+    r = rng.uniform(0.1, 3, len(rows)).astype(np.float32)
+    assert np.array_equal(M.calcFwProj(A, x, "exact"), orc.fwproj(ends, el, x))
+    assert np.array_equal(M.calcBkProj(A, r, "exact"), orc.bkproj(ends, el, r, cols))
+    assert rel_linf(M.calcFwProj(A, x, "fast"), orc.f64_fwproj(ends, el, x)) < TOL_PROJ
+    assert rel_linf(M.calcBkProj(A, r, "fast"), orc.f64_bkproj(ends, el, r, cols)) < TOL_PROJ
+    A.close()
+
+
+def test_errors_on_device(tiny):
+    p, A = tiny
+    with pytest.raises(M.MlemError) as e:
+        M.calcFwProj(A, np.zeros(A.columns() + 1, np.float32))
+    assert e.value.code == -1
+    bad = p.matrix.elements.copy()
+    bad["column"][5] = p.matrix.cols
+    with pytest.raises(M.MlemError) as e:
+        M.Csr4Matrix.from_arrays(p.matrix.rows, p.matrix.cols, p.matrix.rowidx, bad)
+    assert e.value.code == -3 and "column index" in str(e.value)
+
+
+def test_no_csc_matrix_atomic_only(tiny):
+    p, _ = tiny
+    A = _open(p.matrix, no_csc=True)
+    x = M.mlem(A, p.y, None, 5, mode="atomic")
+    xo = orc.mlem(p.matrix.rowidx, p.matrix.elements, A.columns(), p.y, 5)[0]
+    assert rel_linf(x, xo) < 1e-5
+    with pytest.raises(M.MlemError):
+        M.mlem(A, p.y, None, 5, mode="fast")
+    A.close()
+
+
+# ------------------------------------------------------------------- CLI ----
+CLI = ROOT / "mlem_amd" / "bin" / "mlem_cli"
+
+
+def _cli(args, cwd, mode="exact"):
+    env = dict(os.environ, MLEM_MODE=mode)
+    return subprocess.run([str(CLI), *map(str, args)], cwd=cwd, env=env, capture_output=True,
+                          text=True, timeout=120)
+
+
+def test_cli_drop_in(tmp_path, tiny_problem):
+    p = tiny_problem
+    mtx, sino, out = tmp_path / "m.csr4", tmp_path / "s.sino", tmp_path / "o.img"
+    p.matrix.write_csr4(str(mtx))
+    p.y.tofile(sino)
+    r = _cli([mtx, sino, out, 6], tmp_path)
+    assert r.returncode == 0, r.stderr
+    x = np.fromfile(out, np.float32)
+    xo, s, z, sub, _ = orc.mlem(p.matrix.rowidx, p.matrix.elements, p.matrix.cols, p.y, 6)
+    assert np.array_equal(x, xo)
+    lines = r.stdout.splitlines()
+    assert lines[0] == f"Matrix file: {mtx}" and lines[3] == "Iterations: 6"
+    assert any(l.startswith("INIT: sumnorm=") for l in lines)
+    fin = [l for l in lines if l.startswith("Finished iteration ")]
+    assert len(fin) == 6 and fin[0].startswith("Finished iteration 1,  image sum: ")
+    # the printed image sums are the sequential float sums of the reference
+    printed = [float(l.split("image sum: ")[1].split(",")[0]) for l in fin]
+    assert np.allclose(printed, s, rtol=1e-5)
+    assert any(l.startswith("Zeros: ") for l in lines)
+
+
+def test_cli_size_mismatch(tmp_path, tiny_problem):
+    p = tiny_problem
+    mtx, sino = tmp_path / "m.csr4", tmp_path / "s.sino"
+    p.matrix.write_csr4(str(mtx))
+    p.y[:-1].tofile(sino)
+    r = _cli([mtx, sino, tmp_path / "o.img", 2], tmp_path)
+    assert r.returncode == 1 and "rows" in r.stderr
+
+
+def test_cli_checkpoint_restart(tmp_path, tiny_problem):
+    p = tiny_problem
+    mtx, sino = tmp_path / "m.csr4", tmp_path / "s.sino"
+    p.matrix.write_csr4(str(mtx))
+    p.y.tofile(sino)
+    full = tmp_path / "full.img"
+    assert _cli([mtx, sino, full, 7], tmp_path).returncode == 0
+    # a complete checkpoint after 3 iterations, then restart to 7
+    xo3 = orc.mlem(p.matrix.rowidx, p.matrix.elements, p.matrix.cols, p.y, 3)[0]
+    xo3.tofile(tmp_path / "img.chkpt")
+    np.int32([3]).tofile(tmp_path / "iter.chkpt")
+    out = tmp_path / "restart.img"
+    r = _cli([mtx, sino, out, 7, 2], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "I got Checkpoint at iteration 3" in r.stdout
+    assert np.array_equal(np.fromfile(out, np.float32), np.fromfile(full, np.float32))
+    assert not (tmp_path / "img.chkpt").exists()      # removed on clean exit
+    # incomplete checkpoint is ignored
+    xo3.tofile(tmp_path / "img.chkpt")
+    r = _cli([mtx, sino, out, 7, 2], tmp_path)
+    assert r.returncode == 0 and "omitted incomplete checkpoint" in r.stdout
+    assert np.array_equal(np.fromfile(out, np.float32), np.fromfile(full, np.float32))
+    # wrong-size checkpoint is an error
+    np.zeros(5, np.float32).tofile(tmp_path / "img.chkpt")
+    np.int32([3]).tofile(tmp_path / "iter.chkpt")
+    r = _cli([mtx, sino, out, 7, 2], tmp_path)
+    assert r.returncode == 1 and "Mismatch element number" in r.stderr
+
+
+def test_cli_writes_checkpoints(tmp_path, tiny_problem):
+    p = tiny_problem
+    mtx, sino = tmp_path / "m.csr4", tmp_path / "s.sino"
+    p.matrix.write_csr4(str(mtx))
+    p.y.tofile(sino)
+    # the checkpoint files exist while running and are deleted at the end;
+    # check their content through the library, which writes them the same way
+    chk = tmp_path / "chk"
+    chk.mkdir()
+    A = _open(p.matrix)
+    x = M.mlem(A, p.y, None, 4, mode="exact", checkpoint_interval=2, checkpoint_dir=str(chk))
+    assert not (chk / "img.chkpt").exists() and not (chk / "iter.chkpt").exists()
+    assert np.array_equal(x, orc.mlem(p.matrix.rowidx, p.matrix.elements, p.matrix.cols, p.y, 4)[0])
+    A.close()
