@@ -67,7 +67,12 @@ def main() -> int:
     ap.add_argument("--cpu-iters", type=int, default=4,
                     help="iterations of the CPU baseline sample (0 = skip)")
     ap.add_argument("--autotune", action="store_true")
-    ap.add_argument("--launch", default="", help="fwd_lanes,fwd_bpc,bwd_lanes,bwd_bpc")
+    ap.add_argument("--keep-order", action="store_true",
+                    help="keep file row/voxel order on the device (no locality layout)")
+    ap.add_argument("--launch", default="", help="family,param,family,param e.g. tiles,8,groups,32")
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="first run the plain copy kernel over this many MiB (calibrates "
+                         "hardware byte counters under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -77,17 +82,24 @@ def main() -> int:
     import mlem_amd as M
     from mlem_amd import synth
 
+    if args.calibration_copy_mib > 0:
+        nbytes = args.calibration_copy_mib << 20
+        ms = M.calibration_copy(nbytes, 3)
+        print(f"[bench] copy kernel: {nbytes} B read + {nbytes} B written in {ms:.3f} ms "
+              f"({2 * nbytes / ms / 1e6:.0f} GB/s)", file=sys.stderr, flush=True)
+
     t0 = time.time()
     prob = synth.make_problem(args.workload)
     m = prob.matrix
     print(f"[bench] workload {args.workload}: {m.rows} rows, {m.cols} cols, {m.nnz} nnz "
           f"(generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     t0 = time.time()
-    A = M.Csr4Matrix.from_arrays(m.rows, m.cols, m.rowidx, m.elements,
-                                 no_csc=(args.mode == "atomic"))
+    A = M.Csr4Matrix.from_arrays(m.rows, m.cols, m.rowidx, m.elements, grid=m.grid,
+                                 no_csc=(args.mode == "atomic"), keep_order=args.keep_order)
     print(f"[bench] matrix in HBM + CSC in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     if args.launch:
-        A.set_launch(*[int(v) for v in args.launch.split(",")])
+        f, fp, b, bp = args.launch.split(",")
+        A.set_launch((f, int(fp)), (b, int(bp)))
     if args.autotune:
         A.autotune()
     launch = A.launch()

@@ -57,6 +57,8 @@ extern "C" {
 
 /* ---- open flags ------------------------------------------------------ */
 #define MLEM_OPEN_NO_CSC 0x1u /* skip the CSC build (ATOMIC mode only)   */
+#define MLEM_OPEN_KEEP_ORDER 0x2u /* keep file row/voxel order on the device
+                                     (no locality layout)                   */
 
 typedef struct mlem_matrix mlem_matrix; /* device-resident system matrix */
 typedef struct mlem_recon mlem_recon;   /* one reconstruction's state    */
@@ -92,11 +94,14 @@ MLEM_API int mlem_probe(const char* path, uint32_t* rows, uint32_t* cols,
  * csr4 row index (nRows u64 exclusive END offsets, csr4matrix.cpp:141-147,
  * csr4matrix.hpp:138-149) and elements holds nnz 8-byte {u32 column,
  * f32 value} records (helper_files_common/matrixelement.hpp:37-46).
+ * grid (may be NULL) gives the voxel grid nx, ny, nz with voxel number
+ * x + y*nx + z*nx*ny (helper_files_common/scannerconfig.cpp:30-34); when
+ * nx*ny*nz == cols the device copy uses the locality layout.
  * Counterpart of constructing a Csr4Matrix and installing the element array
  * with Csr4Matrix::setData (csr4matrix.hpp:46). */
 MLEM_API int mlem_open_host(uint32_t rows, uint32_t cols, uint64_t nnz,
                    const uint64_t* rowidx, const void* elements,
-                   unsigned flags, mlem_matrix** out);
+                   const uint32_t grid[3], unsigned flags, mlem_matrix** out);
 
 /* Release device memory, stream and handle. Replaces Csr4Matrix::~Csr4Matrix
  * (csr4matrix.cpp:159-163). Accepts NULL. */
@@ -201,18 +206,26 @@ MLEM_API int mlem_recon_image(mlem_recon* r, float* x, uint32_t cols);
 MLEM_API int mlem_recon_iterations(const mlem_recon* r);
 
 /* ---- launch configuration (autotuner) --------------------------------------
- * lanes: lanes per row (forward) / per column (backward) in FAST mode, one of
- * 4, 8, 16, 32, 64; blocks_per_cu: 0 = one 256-lane block per 256/lanes
- * rows, else a grid of blocks_per_cu * CUs blocks walking contiguous chunks.
- * Defaults come from the average row/column length. No reference
- * counterpart (the reference hard-codes 1024-thread blocks,
+ * FAST-mode kernel family and its parameter, per projection:
+ *   MLEM_KERNEL_TILES  param = elements per lane (4, 8 or 16): 256-lane
+ *                      blocks stream tiles of 256*param elements made of
+ *                      whole rows (columns) and reduce them in LDS (default,
+ *                      param 8);
+ *   MLEM_KERNEL_GROUPS param = lanes per row (column): 4, 8, 16, 32 or 64.
+ *   MLEM_KERNEL_DICT   param = elements per lane (8 or 16): tiles whose
+ *                      distinct gather indices are stored as a sorted
+ *                      per-tile dictionary (built on the device on first
+ *                      use); each distinct x (r) value is gathered once
+ *                      into LDS.
+ * No reference counterpart (the reference hard-codes 1024-thread blocks,
  * mlem_cuda/versions/version_5.cu:121-126). */
-MLEM_API int mlem_get_launch(const mlem_matrix* m, int* fwd_lanes,
-                             int* fwd_blocks_per_cu, int* bwd_lanes,
-                             int* bwd_blocks_per_cu);
-MLEM_API int mlem_set_launch(mlem_matrix* m, int fwd_lanes,
-                             int fwd_blocks_per_cu, int bwd_lanes,
-                             int bwd_blocks_per_cu);
+#define MLEM_KERNEL_GROUPS 0
+#define MLEM_KERNEL_TILES 1
+#define MLEM_KERNEL_DICT 2
+MLEM_API int mlem_get_launch(const mlem_matrix* m, int* fwd_kernel,
+                             int* fwd_param, int* bwd_kernel, int* bwd_param);
+MLEM_API int mlem_set_launch(mlem_matrix* m, int fwd_kernel, int fwd_param,
+                             int bwd_kernel, int bwd_param);
 /* Time every forward and backward configuration with HIP events on this
  * matrix and keep the fastest of each. */
 MLEM_API int mlem_autotune(mlem_matrix* m);

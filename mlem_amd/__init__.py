@@ -25,7 +25,7 @@ from typing import Optional
 import numpy as np
 
 from ._native import (INIT_GIVEN, INIT_REFERENCE, MODE_ATOMIC, MODE_EXACT, MODE_FAST, MODES,
-                      OPEN_NO_CSC, MlemError, Options, check, lib)
+                      OPEN_KEEP_ORDER, OPEN_NO_CSC, MlemError, Options, check, lib)
 
 __all__ = [
     "Csr4Matrix", "Recon", "MlemError", "ELEMENT_DTYPE", "probe", "calcColumnSums", "initImage",
@@ -55,6 +55,10 @@ def _i32(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.int32)
 
 
+def _flags(no_csc: bool, keep_order: bool) -> int:
+    return (OPEN_NO_CSC if no_csc else 0) | (OPEN_KEEP_ORDER if keep_order else 0)
+
+
 def probe(path: str) -> dict:
     """Header of a csr4 file (no GPU needed)."""
     rows, cols, grid, sym = C.c_uint32(), C.c_uint32(), (C.c_uint32 * 3)(), C.c_uint32()
@@ -72,7 +76,8 @@ class Csr4Matrix:
     Csr4Matrix.from_arrays(rows, cols, rowidx, elements)   from host memory
     """
 
-    def __init__(self, path: Optional[str] = None, *, no_csc: bool = False, _handle=None):
+    def __init__(self, path: Optional[str] = None, *, no_csc: bool = False,
+                 keep_order: bool = False, _handle=None):
         self._h = None
         if _handle is not None:
             self._h = _handle
@@ -80,7 +85,7 @@ class Csr4Matrix:
             if path is None:
                 raise ValueError("path required")
             h = C.c_void_p()
-            check(lib().mlem_open(os.fsencode(path), OPEN_NO_CSC if no_csc else 0, C.byref(h)))
+            check(lib().mlem_open(os.fsencode(path), _flags(no_csc, keep_order), C.byref(h)))
             self._h = h
         r, c, n = C.c_uint32(), C.c_uint32(), C.c_uint64()
         check(lib().mlem_dims(self._h, C.byref(r), C.byref(c), C.byref(n)))
@@ -88,7 +93,8 @@ class Csr4Matrix:
 
     @classmethod
     def from_arrays(cls, rows: int, cols: int, rowidx: np.ndarray, elements: np.ndarray,
-                    no_csc: bool = False) -> "Csr4Matrix":
+                    grid=None, no_csc: bool = False, keep_order: bool = False) -> "Csr4Matrix":
+        """grid = (nx, ny, nz) enables the device locality layout."""
         rowidx = np.ascontiguousarray(rowidx, dtype=np.uint64)
         if elements.dtype != ELEMENT_DTYPE:
             elements = np.ascontiguousarray(elements).view(ELEMENT_DTYPE)
@@ -96,9 +102,10 @@ class Csr4Matrix:
         if rowidx.shape != (rows,):
             raise ValueError("rowidx must hold one END offset per row")
         nnz = int(elements.shape[0])
+        g = None if grid is None else (C.c_uint32 * 3)(*grid)
         h = C.c_void_p()
-        check(lib().mlem_open_host(rows, cols, nnz, _ptr(rowidx), _ptr(elements),
-                                   OPEN_NO_CSC if no_csc else 0, C.byref(h)))
+        check(lib().mlem_open_host(rows, cols, nnz, _ptr(rowidx), _ptr(elements), g,
+                                   _flags(no_csc, keep_order), C.byref(h)))
         return cls(_handle=h)
 
     def rows(self) -> int:
@@ -116,14 +123,18 @@ class Csr4Matrix:
         return tuple(grid), sym.value
 
     def launch(self) -> dict:
+        """FAST-mode kernels: {"fwd": (family, param), "bwd": (family, param)},
+        family "tiles" (param = elements per lane) or "groups" (lanes/row)."""
         v = [C.c_int() for _ in range(4)]
         check(lib().mlem_get_launch(self.handle, *[C.byref(a) for a in v]))
-        return {"fwd_lanes": v[0].value, "fwd_blocks_per_cu": v[1].value,
-                "bwd_lanes": v[2].value, "bwd_blocks_per_cu": v[3].value}
+        fam = {0: "groups", 1: "tiles", 2: "dict"}
+        return {"fwd": (fam[v[0].value], v[1].value), "bwd": (fam[v[2].value], v[3].value)}
 
-    def set_launch(self, fwd_lanes, fwd_blocks_per_cu, bwd_lanes, bwd_blocks_per_cu) -> None:
-        check(lib().mlem_set_launch(self.handle, fwd_lanes, fwd_blocks_per_cu, bwd_lanes,
-                                    bwd_blocks_per_cu))
+    def set_launch(self, fwd, bwd) -> None:
+        """fwd, bwd: (family, param) as returned by launch()."""
+        fam = {"groups": 0, "tiles": 1, "dict": 2}
+        check(lib().mlem_set_launch(self.handle, fam[fwd[0]], int(fwd[1]), fam[bwd[0]],
+                                    int(bwd[1])))
 
     def autotune(self) -> dict:
         check(lib().mlem_autotune(self.handle))

@@ -26,6 +26,7 @@ MODE_ATOMIC = 2
 MODES = {"fast": MODE_FAST, "exact": MODE_EXACT, "atomic": MODE_ATOMIC}
 
 OPEN_NO_CSC = 0x1
+OPEN_KEEP_ORDER = 0x2
 INIT_REFERENCE = 0
 INIT_GIVEN = 1
 
@@ -61,7 +62,7 @@ SIGNATURES = {
     "mlem_version": (C.c_char_p, []),
     "mlem_probe": (C.c_int, [C.c_char_p, _PU32, _PU32, C.POINTER(C.c_uint64), _PU32, _PU32]),
     "mlem_open": (C.c_int, [C.c_char_p, C.c_uint, C.POINTER(_P)]),
-    "mlem_open_host": (C.c_int, [_u32, _u32, _u64, _P, _P, C.c_uint, C.POINTER(_P)]),
+    "mlem_open_host": (C.c_int, [_u32, _u32, _u64, _P, _P, _P, C.c_uint, C.POINTER(_P)]),
     "mlem_close": (None, [_P]),
     "mlem_dims": (C.c_int, [_P, _PU32, _PU32, C.POINTER(C.c_uint64)]),
     "mlem_geometry": (C.c_int, [_P, _PU32, _PU32]),

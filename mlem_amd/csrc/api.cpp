@@ -159,6 +159,15 @@ struct mlem_matrix {
     dfree(m.sf);
     dfree(m.sr);
     dfree(m.sy);
+    dfree(m.tx);
+    dfree(m.tf);
+    dfree(m.colmap);
+    dfree(m.rowold);
+    dfree(m.rowpos);
+    dfree(m.tiles_fwd);
+    dfree(m.tiles_bwd);
+    free_dict(m.dfwd);
+    free_dict(m.dbwd);
     if (m.stream) (void)hipStreamDestroy(m.stream);
   }
 };
@@ -172,8 +181,8 @@ struct mlem_recon {
   int32_t* y = nullptr;
   float* x = nullptr;
   float* r = nullptr;
-  float* u = nullptr;  // ATOMIC mode accumulator
-  float* nscratch = nullptr;
+  float* u = nullptr;   // ATOMIC mode accumulator
+  float* xs = nullptr;  // file-order image staging
   StatsState st;
   std::vector<double> loglik, fwdtot;  // per completed iteration
   hipGraph_t graph = nullptr;
@@ -187,6 +196,7 @@ struct mlem_recon {
     dfree(x);
     dfree(r);
     dfree(u);
+    dfree(xs);
     dfree(st.partials);
     dfree(st.counter);
     dfree(st.results);
@@ -228,54 +238,88 @@ float time_variant(hipStream_t s, Launch&& launch, int reps) {
 
 // Measure the forward and backward kernels for every lane count and keep the
 // fastest. Scratch vectors are filled with 1 so the timing sees real data.
+GroupLaunch make_launch(int kind, int param) {
+  GroupLaunch g;
+  g.kind = kind;
+  if (kind == kTiles) {
+    if (param != 4 && param != 8 && param != 16)
+      raise(MLEM_EINVAL, "tile elements per lane must be 4, 8 or 16");
+    g.ept = param;
+  } else if (kind == kDict) {
+    if (param != 8 && param != 16)
+      raise(MLEM_EINVAL, "dictionary tile elements per lane must be 8 or 16");
+    g.ept = param;
+  } else if (kind == kGroups) {
+    if (param != 4 && param != 8 && param != 16 && param != 32 && param != 64)
+      raise(MLEM_EINVAL, "lanes per row must be 4, 8, 16, 32 or 64");
+    g.lanes = param;
+  } else {
+    raise(MLEM_EINVAL, "unknown kernel family");
+  }
+  return g;
+}
+
+// Candidate FAST-mode configurations of one projection.
+std::vector<GroupLaunch> launch_candidates() {
+  std::vector<GroupLaunch> c;
+  for (int e : {8, 16}) c.push_back(make_launch(kDict, e));
+  for (int e : {4, 8, 16}) c.push_back(make_launch(kTiles, e));
+  for (int g : {8, 16, 32, 64}) c.push_back(make_launch(kGroups, g));
+  return c;
+}
+
+// Time every forward and backward configuration on this matrix with HIP
+// events (median of 3 launches on real-sized vectors) and keep the fastest.
 void autotune(DevMatrix& m) {
   if (m.nnz == 0) return;
-  float *x, *r;
-  int32_t* y;
+  float *x = nullptr, *r = nullptr, *xx = nullptr;
+  int32_t* y = nullptr;
+  struct F {
+    void** p[4];
+    ~F() {
+      for (void** q : p)
+        if (*q) (void)hipFree(*q);
+    }
+  } f{{(void**)&x, (void**)&r, (void**)&y, (void**)&xx}};
   MLEM_HIP(hipMalloc(&x, 4ull * std::max<uint32_t>(m.cols, 1)));
+  MLEM_HIP(hipMalloc(&xx, 4ull * std::max<uint32_t>(m.cols, 1)));
   MLEM_HIP(hipMalloc(&r, 4ull * std::max<uint32_t>(m.rows, 1)));
   MLEM_HIP(hipMalloc(&y, 4ull * std::max<uint32_t>(m.rows, 1)));
-  struct F {
-    void* p[3];
-    ~F() {
-      for (void* q : p) (void)hipFree(q);
-    }
-  } f{{x, r, y}};
   launch_fill(x, 1.f, m.cols, m.stream);
   launch_fill(r, 1.f, m.rows, m.stream);
-  MLEM_HIP(hipMemsetAsync(y, 0, 4ull * m.rows, m.stream));
-  const int lanes[] = {4, 8, 16, 32, 64};
-  const int bpcs[] = {0, 8};
+  launch_fill(reinterpret_cast<float*>(y), 0.f, m.rows, m.stream);
   float best = 1e30f;
   GroupLaunch bestF = m.fwd, bestB = m.bwd;
-  for (int g : lanes)
-    for (int bpc : bpcs) {
-      m.fwd = GroupLaunch{g, bpc};
-      float t = time_variant(m.stream, [&] { launch_fwd(m, MLEM_MODE_FAST, x, y, r, true, nullptr, m.stream); }, 3);
-      if (t < best) {
-        best = t;
-        bestF = m.fwd;
-      }
+  for (const GroupLaunch& g : launch_candidates()) {
+    m.fwd = g;
+    prepare_tiles(m, m.stream);
+    const float t = time_variant(
+        m.stream, [&] { launch_fwd(m, MLEM_MODE_FAST, x, y, r, true, nullptr, m.stream); }, 3);
+    if (t < best) {
+      best = t;
+      bestF = g;
     }
+  }
   m.fwd = bestF;
   if (m.csc) {
     best = 1e30f;
-    float* xx;
-    MLEM_HIP(hipMalloc(&xx, 4ull * std::max<uint32_t>(m.cols, 1)));
-    for (int g : lanes)
-      for (int bpc : bpcs) {
-        m.bwd = GroupLaunch{g, bpc};
-        float t = time_variant(m.stream, [&] { launch_bwd(m, MLEM_MODE_FAST, r, m.norm, xx, false, m.stream); }, 3);
-        if (t < best) {
-          best = t;
-          bestB = m.bwd;
-        }
+    for (const GroupLaunch& g : launch_candidates()) {
+      m.bwd = g;
+      prepare_tiles(m, m.stream);
+      const float t = time_variant(
+          m.stream, [&] { launch_bwd(m, MLEM_MODE_FAST, r, m.norm, xx, false, m.stream); }, 3);
+      if (t < best) {
+        best = t;
+        bestB = g;
       }
-    (void)hipFree(xx);
+    }
     m.bwd = bestB;
   }
+  prepare_tiles(m, m.stream);
   MLEM_HIP(hipStreamSynchronize(m.stream));
 }
+
+void ensure_scratch(DevMatrix& m);
 
 // Shared tail of mlem_open / mlem_open_host once rowptr and csr are on the
 // device: validate columns, build the CSC, column sums, launch tuning.
@@ -298,34 +342,31 @@ void finish_open(mlem_matrix* h, unsigned flags) {
       raise(MLEM_EFORMAT, std::to_string(hbad) + " matrix elements have a column index >= " +
                               std::to_string(m.cols));
   }
+  const bool reorder = !(flags & MLEM_OPEN_KEEP_ORDER);
+  build_layout(m, reorder, reorder, !(flags & MLEM_OPEN_NO_CSC), m.stream);
   MLEM_HIP(hipMalloc(&m.norm, 4ull * std::max<uint32_t>(m.cols, 1)));
-  if (!(flags & MLEM_OPEN_NO_CSC)) {
-    build_csc(m, m.stream);
+  if (m.csc)
     launch_colsums_csc(m, m.norm, m.stream);
-  } else {
+  else
     launch_colsums_atomic(m, m.norm, m.stream);
-  }
+  ensure_scratch(m);
+  to_file_cols(m, m.norm, m.tx, m.stream);
   std::vector<float> norm(m.cols);
   if (m.cols)
-    MLEM_HIP(hipMemcpyAsync(norm.data(), m.norm, 4ull * m.cols, hipMemcpyDeviceToHost, m.stream));
+    MLEM_HIP(hipMemcpyAsync(norm.data(), m.tx, 4ull * m.cols, hipMemcpyDeviceToHost, m.stream));
   MLEM_HIP(hipStreamSynchronize(m.stream));
-  // initImage's sum of the norm: double, index order (seqcsr4mlem.cpp:55-56).
+  // initImage's sum of the norm: double, file index order (seqcsr4mlem.cpp:55-56).
   double sn = 0.0;
   for (uint32_t j = 0; j < m.cols; ++j) sn += norm[j];
   m.sumnorm = sn;
   h->norm_seconds = seconds_since(t0);
 
   const double avgRow = m.rows ? (double)m.nnz / m.rows : 1.0;
-  const double avgCol = m.cols ? (double)m.nnz / m.cols : 1.0;
-  m.fwd = GroupLaunch{lanes_for(avgRow), 0};
-  m.bwd = GroupLaunch{lanes_for(avgCol), 0};
-  m.scatter = GroupLaunch{lanes_for(avgRow), 0};
+  m.fwd = make_launch(kTiles, 8);
+  m.bwd = make_launch(kTiles, 8);
+  m.scatter = make_launch(kGroups, lanes_for(avgRow));
+  prepare_tiles(m, m.stream);
   if (env_int("MLEM_AUTOTUNE", 0) > 0) autotune(m);
-  m.fwd.lanes = env_int("MLEM_FWD_LANES", m.fwd.lanes);
-  m.fwd.blocks_per_cu = env_int("MLEM_FWD_BPC", m.fwd.blocks_per_cu);
-  m.bwd.lanes = env_int("MLEM_BWD_LANES", m.bwd.lanes);
-  m.bwd.blocks_per_cu = env_int("MLEM_BWD_BPC", m.bwd.blocks_per_cu);
-  m.scatter.lanes = env_int("MLEM_SCATTER_LANES", m.scatter.lanes);
 }
 
 mlem_matrix* new_matrix(uint32_t rows, uint32_t cols, uint64_t nnz) {
@@ -369,6 +410,8 @@ void ensure_scratch(DevMatrix& m) {
   if (!m.sf) MLEM_HIP(hipMalloc(&m.sf, 4ull * std::max<uint32_t>(m.rows, 1)));
   if (!m.sr) MLEM_HIP(hipMalloc(&m.sr, 4ull * std::max<uint32_t>(m.rows, 1)));
   if (!m.sy) MLEM_HIP(hipMalloc(&m.sy, 4ull * std::max<uint32_t>(m.rows, 1)));
+  if (!m.tx) MLEM_HIP(hipMalloc(&m.tx, 4ull * std::max<uint32_t>(m.cols, 1)));
+  if (!m.tf) MLEM_HIP(hipMalloc(&m.tf, 4ull * std::max<uint32_t>(m.rows, 1)));
 }
 
 void h2d(void* d, const void* h, size_t n, hipStream_t s) {
@@ -456,6 +499,7 @@ mlem_recon* create_recon(mlem_matrix* h, const int32_t* y, uint32_t rows, const 
   MLEM_HIP(hipMalloc(&R->y, 4ull * std::max<uint32_t>(rows, 1)));
   MLEM_HIP(hipMalloc(&R->x, 4ull * std::max<uint32_t>(cols, 1)));
   MLEM_HIP(hipMalloc(&R->r, 4ull * std::max<uint32_t>(rows, 1)));
+  MLEM_HIP(hipMalloc(&R->xs, 4ull * std::max<uint32_t>(cols, 1)));
   if (R->mode == MLEM_MODE_ATOMIC) {
     MLEM_HIP(hipMalloc(&R->u, 4ull * std::max<uint32_t>(cols, 1)));
     launch_fill(R->u, 0.f, cols, m.stream);
@@ -468,14 +512,18 @@ mlem_recon* create_recon(mlem_matrix* h, const int32_t* y, uint32_t rows, const 
     MLEM_HIP(hipMalloc(&R->st.results, sizeof(double2) * kStatsRing));
     MLEM_HIP(hipMemsetAsync(R->st.counter, 0, 2 * sizeof(unsigned), m.stream));
   }
-  h2d(R->y, y, 4ull * rows, m.stream);
+  // sinogram into device row order (r is free until the first iteration)
+  int32_t* ystage = reinterpret_cast<int32_t*>(R->r);
+  h2d(ystage, y, 4ull * rows, m.stream);
+  rows_to_device(m, (const int32_t*)ystage, R->y, m.stream);
   // initImage (seqcsr4mlem.cpp:52-66): sum of int32 counts in double is exact
   // in any order; the norm's double sum was taken in index order at open.
   double sumin = 0.0;
   for (uint32_t i = 0; i < rows; ++i) sumin += y[i];
   if (sumin_out) *sumin_out = sumin;
   if (o->init == MLEM_INIT_GIVEN) {
-    h2d(R->x, x0, 4ull * cols, m.stream);
+    h2d(R->xs, x0, 4ull * cols, m.stream);
+    to_device_cols(m, R->xs, R->x, m.stream);
   } else {
     const float initial = (float)(sumin / m.sumnorm);
     if (initial_out) *initial_out = initial;
@@ -483,6 +531,13 @@ mlem_recon* create_recon(mlem_matrix* h, const int32_t* y, uint32_t rows, const 
   }
   MLEM_HIP(hipStreamSynchronize(m.stream));
   return R.release();
+}
+
+// Current image in file voxel order into host memory (enqueued, not synced).
+void read_image(mlem_recon* R, float* host) {
+  DevMatrix& m = R->mat->m;
+  to_file_cols(m, R->x, R->xs, m.stream);
+  d2h(host, R->xs, 4ull * m.cols, m.stream);
 }
 
 bool file_exists(const std::string& p) {
@@ -553,13 +608,16 @@ int mlem_probe(const char* path, uint32_t* rows, uint32_t* cols, uint64_t* nnz, 
 }
 
 int mlem_open_host(uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t* rowidx,
-                   const void* elements, unsigned flags, mlem_matrix** out) {
+                   const void* elements, const uint32_t grid[3], unsigned flags,
+                   mlem_matrix** out) {
   return guarded([&] {
     if (!out || (rows && !rowidx) || (nnz && !elements)) raise(MLEM_EINVAL, "null argument");
     *out = nullptr;
     csr4_check_rowidx(rowidx, rows, nnz);
     auto t0 = std::chrono::steady_clock::now();
     std::unique_ptr<mlem_matrix> h(new_matrix(rows, cols, nnz));
+    if (grid)
+      for (int k = 0; k < 3; ++k) h->m.grid[k] = grid[k];
     upload_rowidx(h->m, rowidx);
     const char* src = static_cast<const char*>(elements);
     stream_to_device(h->m.csr, 8ull * nnz, h->m.stream,
@@ -600,7 +658,9 @@ int mlem_column_sums(mlem_matrix* h, float* norm, uint32_t cols) {
   return guarded([&] {
     check_dims(h, h ? h->m.rows : 0, cols);
     if (!norm && cols) raise(MLEM_EINVAL, "null output");
-    d2h(norm, h->m.norm, 4ull * cols, h->m.stream);
+    ensure_scratch(h->m);
+    to_file_cols(h->m, h->m.norm, h->m.tx, h->m.stream);
+    d2h(norm, h->m.tx, 4ull * cols, h->m.stream);
     MLEM_HIP(hipStreamSynchronize(h->m.stream));
   });
 }
@@ -631,10 +691,12 @@ int mlem_fwproj(mlem_matrix* h, const float* x, uint32_t cols, float* f, uint32_
     if ((cols && !x) || (rows && !f)) raise(MLEM_EINVAL, "null argument");
     DevMatrix& m = h->m;
     ensure_scratch(m);
-    h2d(m.sx, x, 4ull * cols, m.stream);
+    h2d(m.tx, x, 4ull * cols, m.stream);
+    to_device_cols(m, m.tx, m.sx, m.stream);
     launch_fwd(m, mode == MLEM_MODE_EXACT ? MLEM_MODE_EXACT : MLEM_MODE_FAST, m.sx, nullptr, m.sf,
                false, nullptr, m.stream);
-    d2h(f, m.sf, 4ull * rows, m.stream);
+    rows_to_file(m, (const float*)m.sf, m.tf, m.stream);
+    d2h(f, m.tf, 4ull * rows, m.stream);
     MLEM_HIP(hipStreamSynchronize(m.stream));
   });
 }
@@ -659,14 +721,16 @@ int mlem_bkproj(mlem_matrix* h, const float* r, uint32_t rows, float* u, uint32_
     if ((rows && !r) || (cols && !u)) raise(MLEM_EINVAL, "null argument");
     DevMatrix& m = h->m;
     ensure_scratch(m);
-    h2d(m.sr, r, 4ull * rows, m.stream);
+    h2d(m.tf, r, 4ull * rows, m.stream);
+    rows_to_device(m, (const float*)m.tf, m.sr, m.stream);
     if (mode == MLEM_MODE_ATOMIC) {
       launch_fill(m.su, 0.f, cols, m.stream);
       launch_scatter(m, m.sr, m.su, m.stream);
     } else {
       launch_bwd(m, mode, m.sr, m.norm, m.su, false, m.stream);
     }
-    d2h(u, m.su, 4ull * cols, m.stream);
+    to_file_cols(m, m.su, m.tx, m.stream);
+    d2h(u, m.tx, 4ull * cols, m.stream);
     MLEM_HIP(hipStreamSynchronize(m.stream));
   });
 }
@@ -677,14 +741,7 @@ int mlem_update(mlem_matrix* h, const float* u, const float* norm, float* x, uin
     if (cols && (!u || !norm || !x)) raise(MLEM_EINVAL, "null argument");
     DevMatrix& m = h->m;
     ensure_scratch(m);
-    float* dn = reinterpret_cast<float*>(m.sf);  // borrow: rows may be < cols
-    std::unique_ptr<float, void (*)(float*)> own(nullptr, [](float* p) { (void)hipFree(p); });
-    if (m.rows < cols) {
-      float* p = nullptr;
-      MLEM_HIP(hipMalloc(&p, 4ull * cols));
-      own.reset(p);
-      dn = p;
-    }
+    float* dn = m.tx;  // elementwise: file order throughout
     h2d(m.su, u, 4ull * cols, m.stream);
     h2d(dn, norm, 4ull * cols, m.stream);
     h2d(m.sx, x, 4ull * cols, m.stream);
@@ -767,7 +824,7 @@ int mlem_recon_image(mlem_recon* R, float* x, uint32_t cols) {
     if (!R) raise(MLEM_EINVAL, "null handle");
     check_dims(R->mat, R->mat->m.rows, cols);
     if (cols && !x) raise(MLEM_EINVAL, "null output");
-    d2h(x, R->x, 4ull * cols, R->mat->m.stream);
+    read_image(R, x);
     MLEM_HIP(hipStreamSynchronize(R->mat->m.stream));
   });
 }
@@ -831,7 +888,7 @@ int mlem_reconstruct(mlem_matrix* h, const int32_t* y, uint32_t rows, float* x, 
       for (int iter = start; iter < iterations; ++iter) {
         run_iterations(R.get(), 1);
         const double el = seconds_since(t0);
-        d2h(img.data(), R->x, 4ull * cols, h->m.stream);
+        read_image(R.get(), img.data());
         MLEM_HIP(hipStreamSynchronize(h->m.stream));
         if (verbose) {
           // per-iteration statistics of seqcsr4mlem.cpp:172-185
@@ -857,7 +914,7 @@ int mlem_reconstruct(mlem_matrix* h, const int32_t* y, uint32_t rows, float* x, 
         t0 = std::chrono::steady_clock::now();
       }
     }
-    d2h(x, R->x, 4ull * cols, h->m.stream);
+    read_image(R.get(), x);
     MLEM_HIP(hipStreamSynchronize(h->m.stream));
     const int ran = (int)R->loglik.size();
     for (int k = 0; k < ran; ++k) {
@@ -871,24 +928,24 @@ int mlem_reconstruct(mlem_matrix* h, const int32_t* y, uint32_t rows, float* x, 
   });
 }
 
-int mlem_get_launch(const mlem_matrix* h, int* fl, int* fb, int* bl, int* bb) {
+int mlem_get_launch(const mlem_matrix* h, int* fk, int* fp, int* bk, int* bp) {
   return guarded([&] {
     if (!h) raise(MLEM_EINVAL, "null matrix handle");
-    if (fl) *fl = h->m.fwd.lanes;
-    if (fb) *fb = h->m.fwd.blocks_per_cu;
-    if (bl) *bl = h->m.bwd.lanes;
-    if (bb) *bb = h->m.bwd.blocks_per_cu;
+    const GroupLaunch &f = h->m.fwd, &b = h->m.bwd;
+    if (fk) *fk = f.kind;
+    if (fp) *fp = f.kind == kGroups ? f.lanes : f.ept;
+    if (bk) *bk = b.kind;
+    if (bp) *bp = b.kind == kGroups ? b.lanes : b.ept;
   });
 }
 
-int mlem_set_launch(mlem_matrix* h, int fl, int fb, int bl, int bb) {
+int mlem_set_launch(mlem_matrix* h, int fk, int fp, int bk, int bp) {
   return guarded([&] {
     if (!h) raise(MLEM_EINVAL, "null matrix handle");
-    auto ok = [](int g) { return g == 4 || g == 8 || g == 16 || g == 32 || g == 64; };
-    if (!ok(fl) || !ok(bl)) raise(MLEM_EINVAL, "lanes must be 4, 8, 16, 32 or 64");
-    if (fb < 0 || bb < 0 || fb > 64 || bb > 64) raise(MLEM_EINVAL, "blocks_per_cu must be 0..64");
-    h->m.fwd = GroupLaunch{fl, fb};
-    h->m.bwd = GroupLaunch{bl, bb};
+    const GroupLaunch f = make_launch(fk, fp), b = make_launch(bk, bp);
+    h->m.fwd = f;
+    h->m.bwd = b;
+    prepare_tiles(h->m, h->m.stream);
   });
 }
 

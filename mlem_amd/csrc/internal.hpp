@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 namespace mlem {
 
@@ -18,10 +19,28 @@ struct ApiError {
 void hip_check(hipError_t e, const char* what);
 #define MLEM_HIP(call) ::mlem::hip_check((call), #call)
 
-// Launch geometry of a row- or column-group kernel.
+// FAST-mode kernel family of a projection.
+enum KernelKind { kGroups = 0, kTiles = 1, kDict = 2 };
+
+// Dictionary-tile storage of one projection (built on the device at load
+// time for the elements-per-lane setting `ept`; 0 = not built).
+struct DictFormat {
+  int ept = 0;
+  uint32_t ntiles = 0;
+  uint4* tiles = nullptr;     // [ntiles+1] {first segment, first element,
+                              //  aligned element start, first dict entry}
+  uint32_t* dict = nullptr;   // distinct gather indices, sorted per tile
+  float* dval = nullptr;      // element values, tile-aligned
+  uint16_t* dlix = nullptr;   // element positions in the tile dictionary
+  uint64_t ndict = 0, nval = 0;
+};
+
+// Launch configuration of one projection (forward or backward).
 struct GroupLaunch {
-  int lanes = 16;          // lanes per row (or column); 1 = exact sequential
-  int blocks_per_cu = 0;   // 0: one block per 256/lanes rows, no chunk loop
+  int kind = kTiles;       // kGroups: lane group per row/column; kTiles: tiles
+  int lanes = 16;          // kGroups: lanes per row (or column)
+  int ept = 8;             // kTiles: elements per lane, tile = 256*ept elements
+  int blocks_per_cu = 0;   // kGroups: 0 = one block per 256/lanes rows
 };
 
 struct DevMatrix {
@@ -38,9 +57,27 @@ struct DevMatrix {
   uint32_t* colptr = nullptr;  // [cols+1]
   uint2* csc = nullptr;        // [nnz] {row, value bits}, rows ascending
   float* norm = nullptr;       // [cols] fp32 column sums, ascending row order
+
+  // Locality layout (nullptr = identity): device column of each file column,
+  // file row of each device row, device row of each file row.
+  uint32_t* colmap = nullptr;  // [cols]
+  uint32_t* rowold = nullptr;  // [rows]
+  uint32_t* rowpos = nullptr;  // [rows]
   double sumnorm = 0.0;        // sequential double sum of norm (initImage)
 
   GroupLaunch fwd, bwd, scatter;
+
+  // Tiled kernels: device-order pointers kept on the host, and the tile
+  // partition (first segment of every tile, plus the end) for the current
+  // elements-per-lane setting.
+  std::vector<uint32_t> h_rowptr, h_colptr;
+  uint2* tiles_fwd = nullptr;
+  uint32_t ntiles_fwd = 0;
+  int tiles_fwd_ept = 0;
+  uint2* tiles_bwd = nullptr;
+  uint32_t ntiles_bwd = 0;
+  int tiles_bwd_ept = 0;
+  DictFormat dfwd, dbwd;
 
   // per-handle scratch for the host-buffer step functions
   float* sx = nullptr;   // [cols]
@@ -48,6 +85,8 @@ struct DevMatrix {
   float* sf = nullptr;   // [rows]
   float* sr = nullptr;   // [rows]
   int32_t* sy = nullptr; // [rows]
+  float* tx = nullptr;   // [cols] file-order staging
+  float* tf = nullptr;   // [rows] file-order staging
 };
 
 // Deterministic FP64 reduction state for the forward pass: per-block partial
@@ -64,6 +103,12 @@ struct StatsState {
 // ---- launchers (kernels.hip) -------------------------------------------
 // Grid size of a statistics-carrying forward launch (partials needed).
 unsigned stats_max_blocks(const DevMatrix& m);
+// Copy the device-order row/column pointers to the host (once, after the
+// layout is built) and (re)build the tile partitions and dictionary formats
+// the launch configuration needs.
+void prepare_tiles(DevMatrix& m, hipStream_t s);
+// Release a dictionary format's device memory.
+void free_dict(DictFormat& d);
 // Forward projection. ratio: out = (f != 0) ? y/f : 0, else out = f.
 void launch_fwd(const DevMatrix& m, int mode, const float* x, const int32_t* y,
                 float* out, bool ratio, const StatsState* stats, hipStream_t s);
@@ -83,13 +128,24 @@ void launch_correl(uint32_t rows, const float* f, const int32_t* y, float* r,
 void launch_colsums_csc(const DevMatrix& m, float* norm, hipStream_t s);
 // norm by atomic scatter (matrices opened without a CSC).
 void launch_colsums_atomic(const DevMatrix& m, float* norm, hipStream_t s);
+// ---- load-time layout (layout.hip) ----------------------------------------
+// Renumber rows (sinogram order) and voxels (4x4x2 bricks) when the grid is
+// known, build the CSC (file row order inside every column) and the
+// device-order CSR. Requires validated column indices.
+void build_layout(DevMatrix& m, bool reorder_rows, bool reorder_cols, bool want_csc,
+                  hipStream_t s);
+// File-order <-> device-order vectors (device buffers, distinct pointers).
+void to_device_cols(const DevMatrix& m, const float* file_order, float* dev, hipStream_t s);
+void to_file_cols(const DevMatrix& m, const float* dev, float* file_order, hipStream_t s);
+template <class T>
+void rows_to_device(const DevMatrix& m, const T* file_order, T* dev, hipStream_t s);
+template <class T>
+void rows_to_file(const DevMatrix& m, const T* dev, T* file_order, hipStream_t s);
 // Count column indices >= cols in the CSR (validation).
 void launch_check_columns(const DevMatrix& m, unsigned* bad, hipStream_t s);
 // Build rowptr (u32) from the csr4 END-offset row index already on device.
 void launch_rowptr_from_rowidx(const uint64_t* rowidx, uint32_t rows,
                                uint32_t* rowptr, hipStream_t s);
-// Build colptr/csc from the CSR (stable: rows ascending in every column).
-void build_csc(DevMatrix& m, hipStream_t s);
 // Fill a float buffer.
 void launch_fill(float* p, float v, size_t n, hipStream_t s);
 // Streaming copy kernel (byte-counter calibration).

@@ -24,8 +24,6 @@
 // Launch mapping: workgroup b runs on XCD b % 8; logical_block() renumbers
 // so that each XCD owns one contiguous eighth of the rows (columns), which
 // keeps the CSC pass's r gathers inside one XCD's L2.
-#include <rocprim/device/device_radix_sort.hpp>
-
 #include "internal.hpp"
 #include "mlem.h"
 
@@ -213,6 +211,383 @@ __global__ __launch_bounds__(kBlock) void k_bwd(const uint32_t* __restrict__ col
   }
 }
 
+// ------------------------------------------------------------ tiled SpMV --
+// A tile is a run of whole segments (rows, or CSC columns) holding at most
+// T = 256*EPT elements and at most T segments, or one longer segment alone
+// (partitioned once at load time). A 256-lane block streams the tile's
+// element range with EPT independent coalesced loads per lane, issues all
+// its gathers, writes the products to LDS and sums every segment there with
+// 16-lane groups; the epilogue (ratio or multiplicative update) runs on the
+// segment sum. All of a tile's loads are in flight at once, which is what
+// keeps HBM busy: short segments no longer serialise load -> gather ->
+// reduce per wave.
+// Loads and gathers are unconditional (indices clamped to the tile's last
+// element; the duplicates are masked or never read), so the compiler keeps
+// all EPT element loads, then all EPT gathers, in flight together instead of
+// wrapping each in a branch that waits for the previous one. A tile holds at
+// most 256 segments: lane t owns segment t, fetches its end offset and its
+// epilogue operands (Epi::prefetch) together with the element loads, and
+// finishes it (Epi::finish) from the LDS sum, so no load sits on the tail.
+template <int EPT, class Epi>
+__device__ __forceinline__ void tile_spmv(const uint32_t* __restrict__ segptr,
+                                          const uint2* __restrict__ el,
+                                          const uint2* __restrict__ tiles, uint32_t tile,
+                                          const float* __restrict__ v, Epi& epi) {
+  constexpr uint32_t T = kBlock * EPT;
+  constexpr int G = 16;
+  __shared__ float P[T];
+  __shared__ uint32_t S[kBlock + 1];
+  __shared__ float Sum[kBlock];
+  __shared__ float red[kBlock / kWave];
+  const uint32_t t = threadIdx.x;
+  const uint2 d0 = tiles[tile], d1 = tiles[tile + 1];  // {first segment, first element}
+  const uint32_t s0 = d0.x, e0 = d0.y, s1 = d1.x, e1 = d1.y;
+  const uint32_t nseg = s1 - s0;                        // <= kBlock
+  if (t < nseg) epi.prefetch(s0 + t);
+  if (e1 - e0 > T) {
+    // a single segment longer than a tile: strided partial sums, block sum
+    float acc = 0.f;
+    for (uint32_t base = e0; base < e1; base += T) {
+      u32x2 e[EPT];
+      float g[EPT];
+#pragma unroll
+      for (int j = 0; j < EPT; ++j) e[j] = ld_stream(el + min(base + t + j * kBlock, e1 - 1));
+#pragma unroll
+      for (int j = 0; j < EPT; ++j) g[j] = v[e[j].x];
+#pragma unroll
+      for (int j = 0; j < EPT; ++j) {
+        const float p = __uint_as_float(e[j].y) * g[j];
+        acc += (base + t + j * kBlock < e1) ? p : 0.f;
+      }
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (t % kWave == 0) red[t / kWave] = acc;
+    __syncthreads();
+    if (t == 0) {
+      float sum = 0.f;
+      for (int w = 0; w < kBlock / kWave; ++w) sum += red[w];
+      epi.finish(s0, sum);
+    }
+    return;
+  }
+  uint32_t send = 0;
+  if (t < nseg) send = segptr[s0 + t + 1] - e0;
+  if (e1 > e0) {  // block-uniform
+    u32x2 e[EPT];
+    float g[EPT];
+    const uint32_t last = e1 - 1;
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) e[j] = ld_stream(el + min(e0 + t + j * kBlock, last));
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) g[j] = v[e[j].x];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) P[t + j * kBlock] = __uint_as_float(e[j].y) * g[j];
+  }
+  if (t < nseg) S[t + 1] = send;
+  if (t == 0) S[0] = 0;
+  __syncthreads();
+  const uint32_t lane = t % G, grp = t / G;
+  for (uint32_t sg = grp; sg < nseg; sg += kBlock / G) {
+    const uint32_t b = S[sg], end = S[sg + 1];
+    float acc = 0.f;
+    for (uint32_t k = b + lane; k < end; k += G) acc += P[k];
+    acc = group_sum<G>(acc);
+    if (lane == 0) Sum[sg] = acc;
+  }
+  __syncthreads();
+  if (t < nseg) epi.finish(s0 + t, Sum[t]);
+}
+
+// Forward epilogue: r = (f != 0) ? y/f : 0 (or f itself), optional FP64
+// log-likelihood and sum of f.
+template <bool RATIO, bool STATS>
+struct FwdEpilogue {
+  const int32_t* __restrict__ y;
+  float* __restrict__ out;
+  double ll = 0.0, tot = 0.0;
+  int32_t yi = 0;
+  __device__ void prefetch(uint32_t row) {
+    if constexpr (RATIO) yi = y[row];
+  }
+  __device__ void finish(uint32_t row, float f) {
+    if constexpr (RATIO) {
+      out[row] = (f != 0.f) ? (float)yi / f : 0.f;
+      if constexpr (STATS) {
+        tot += (double)f;
+        if (f > 0.f) ll += (yi != 0 ? (double)yi * log((double)f) : 0.0) - (double)f;
+      }
+    } else {
+      out[row] = f;
+    }
+  }
+};
+
+// Backward epilogue: x *= (n != 0) ? u/n : u (or u itself).
+template <bool UPDATE>
+struct BwdEpilogue {
+  const float* __restrict__ norm;
+  float* __restrict__ out;
+  float n = 0.f, x = 0.f;
+  __device__ void prefetch(uint32_t col) {
+    if constexpr (UPDATE) {
+      n = norm[col];
+      x = out[col];
+    }
+  }
+  __device__ void finish(uint32_t col, float u) {
+    if constexpr (UPDATE)
+      out[col] = x * ((n != 0.f) ? u / n : u);
+    else
+      out[col] = u;
+  }
+};
+
+template <int EPT, bool RATIO, bool STATS>
+__global__ __launch_bounds__(kBlock) void k_fwd_tiles(
+    const uint32_t* __restrict__ rowptr, const uint2* __restrict__ csr,
+    const uint2* __restrict__ tiles, uint32_t ntiles, const float* __restrict__ x,
+    const int32_t* __restrict__ y, float* __restrict__ out, double2* __restrict__ partials,
+    unsigned* __restrict__ counter, double2* __restrict__ results, unsigned ring) {
+  const uint32_t tile = logical_block();
+  FwdEpilogue<RATIO, STATS> epi{y, out};
+  if (tile < ntiles) tile_spmv<EPT>(rowptr, csr, tiles, tile, x, epi);
+  if constexpr (STATS) stats_finish(epi.ll, epi.tot, partials, counter, results, ring);
+}
+
+template <int EPT, bool UPDATE>
+__global__ __launch_bounds__(kBlock) void k_bwd_tiles(const uint32_t* __restrict__ colptr,
+                                                      const uint2* __restrict__ csc,
+                                                      const uint2* __restrict__ tiles,
+                                                      uint32_t ntiles, const float* __restrict__ r,
+                                                      const float* __restrict__ norm,
+                                                      float* __restrict__ out) {
+  const uint32_t tile = logical_block();
+  if (tile >= ntiles) return;
+  BwdEpilogue<UPDATE> epi{norm, out};
+  tile_spmv<EPT>(colptr, csc, tiles, tile, r, epi);
+}
+
+// ------------------------------------------------------- dictionary tiles --
+// The same tiles, stored for gathers that cost less than one cache-line
+// request per element. At load time every tile of at most T elements gets a
+// sorted dictionary of the distinct indices it gathers (voxels for the
+// forward pass, rows for the backward pass) and each element is stored as
+// {fp32 value, u16 position in that dictionary} (SoA, every tile starting on
+// a 64-element boundary). At run time a block gathers each distinct entry
+// once, with sorted (coalescing) addresses, into LDS; elements read their
+// operand from LDS. Tiles of one long segment use the 8-byte records and
+// direct gathers.
+struct DictTileDesc {
+  uint32_t s0, e0, a0, g0;  // first segment, first element, aligned start, first dict entry
+};
+
+template <int EPT>
+__device__ __forceinline__ float long_segment_sum(const uint2* __restrict__ el,
+                                                  const float* __restrict__ v, uint32_t e0,
+                                                  uint32_t e1, float* red) {
+  constexpr uint32_t T = kBlock * EPT;
+  const uint32_t t = threadIdx.x;
+  float acc = 0.f;
+  for (uint32_t base = e0; base < e1; base += T) {
+    u32x2 e[EPT];
+    float g[EPT];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) e[j] = ld_stream(el + min(base + t + j * kBlock, e1 - 1));
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) g[j] = v[e[j].x];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const float p = __uint_as_float(e[j].y) * g[j];
+      acc += (base + t + j * kBlock < e1) ? p : 0.f;
+    }
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if (t % kWave == 0) red[t / kWave] = acc;
+  __syncthreads();
+  float sum = 0.f;
+  for (int w = 0; w < kBlock / kWave; ++w) sum += red[w];
+  return sum;
+}
+
+template <int EPT, class Epi>
+__device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
+                                          const uint2* __restrict__ el,
+                                          const uint4* __restrict__ tiles, uint32_t tile,
+                                          const float* __restrict__ dval,
+                                          const uint16_t* __restrict__ dlix,
+                                          const uint32_t* __restrict__ dict,
+                                          const float* __restrict__ v, Epi& epi) {
+  static_assert(EPT % 2 == 0, "elements are loaded in pairs");
+  constexpr uint32_t T = kBlock * EPT;
+  constexpr int H = EPT / 2;
+  constexpr int G = 16;
+  __shared__ float P[T];
+  __shared__ float V[T];
+  __shared__ uint32_t S[kBlock + 1];
+  __shared__ float Sum[kBlock];
+  __shared__ float red[kBlock / kWave];
+  const uint32_t t = threadIdx.x;
+  const uint4 d0 = tiles[tile], d1 = tiles[tile + 1];
+  const uint32_t s0 = d0.x, e0 = d0.y, a0 = d0.z, g0 = d0.w;
+  const uint32_t nseg = d1.x - s0, count = d1.y - e0, nd = d1.w - g0;
+  if (t < nseg) epi.prefetch(s0 + t);
+  if (count > T) {
+    const float sum = long_segment_sum<EPT>(el, v, e0, d1.y, red);
+    if (t == 0) epi.finish(s0, sum);
+    return;
+  }
+  uint32_t send = 0;
+  if (t < nseg) send = segptr[s0 + t + 1] - e0;
+  u32x2 val[H];
+  uint32_t li[H];
+  if (count > 0) {  // block-uniform
+    const u32x2* dv2 = reinterpret_cast<const u32x2*>(dval + a0);
+    const uint32_t* dl2 = reinterpret_cast<const uint32_t*>(dlix + a0);
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      val[j] = __builtin_nontemporal_load(dv2 + t + j * kBlock);
+      li[j] = __builtin_nontemporal_load(dl2 + t + j * kBlock);
+    }
+    float g[EPT];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) g[j] = v[dict[g0 + min(t + j * kBlock, nd - 1)]];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) V[t + j * kBlock] = g[j];
+  }
+  if (t < nseg) S[t + 1] = send;
+  if (t == 0) S[0] = 0;
+  __syncthreads();
+  if (count > 0) {
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      const uint32_t q = 2 * (t + j * kBlock);
+      P[q] = __uint_as_float(val[j].x) * V[li[j] & 0xFFFFu];
+      P[q + 1] = __uint_as_float(val[j].y) * V[li[j] >> 16];
+    }
+  }
+  __syncthreads();
+  const uint32_t lane = t % G, grp = t / G;
+  for (uint32_t sg = grp; sg < nseg; sg += kBlock / G) {
+    const uint32_t b = S[sg], end = S[sg + 1];
+    float acc = 0.f;
+    for (uint32_t k = b + lane; k < end; k += G) acc += P[k];
+    acc = group_sum<G>(acc);
+    if (lane == 0) Sum[sg] = acc;
+  }
+  __syncthreads();
+  if (t < nseg) epi.finish(s0 + t, Sum[t]);
+}
+
+template <int EPT, bool RATIO, bool STATS>
+__global__ __launch_bounds__(kBlock) void k_fwd_dict(
+    const uint32_t* __restrict__ rowptr, const uint2* __restrict__ csr,
+    const uint4* __restrict__ tiles, uint32_t ntiles, const float* __restrict__ dval,
+    const uint16_t* __restrict__ dlix, const uint32_t* __restrict__ dict,
+    const float* __restrict__ x, const int32_t* __restrict__ y, float* __restrict__ out,
+    double2* __restrict__ partials, unsigned* __restrict__ counter,
+    double2* __restrict__ results, unsigned ring) {
+  const uint32_t tile = logical_block();
+  FwdEpilogue<RATIO, STATS> epi{y, out};
+  if (tile < ntiles) dict_spmv<EPT>(rowptr, csr, tiles, tile, dval, dlix, dict, x, epi);
+  if constexpr (STATS) stats_finish(epi.ll, epi.tot, partials, counter, results, ring);
+}
+
+template <int EPT, bool UPDATE>
+__global__ __launch_bounds__(kBlock) void k_bwd_dict(
+    const uint32_t* __restrict__ colptr, const uint2* __restrict__ csc,
+    const uint4* __restrict__ tiles, uint32_t ntiles, const float* __restrict__ dval,
+    const uint16_t* __restrict__ dlix, const uint32_t* __restrict__ dict,
+    const float* __restrict__ r, const float* __restrict__ norm, float* __restrict__ out) {
+  const uint32_t tile = logical_block();
+  if (tile >= ntiles) return;
+  BwdEpilogue<UPDATE> epi{norm, out};
+  dict_spmv<EPT>(colptr, csc, tiles, tile, dval, dlix, dict, r, epi);
+}
+
+// Dictionary construction, one block per tile. Pass 1 (tiles4 == nullptr)
+// counts the distinct indices of every tile; pass 2 writes the dictionary
+// and the {value, position} element arrays. Keys (index << 16 | element
+// position) are bitonic-sorted in LDS; a block scan of "new index" flags
+// gives every element its dictionary position.
+template <int EPT>
+__global__ __launch_bounds__(kBlock) void k_dict_build(const uint2* __restrict__ el,
+                                                       const uint2* __restrict__ tiles2,
+                                                       const uint4* __restrict__ tiles4,
+                                                       uint32_t* __restrict__ nd_out,
+                                                       uint32_t* __restrict__ dict,
+                                                       float* __restrict__ dval,
+                                                       uint16_t* __restrict__ dlix) {
+  constexpr uint32_t T = kBlock * EPT;
+  __shared__ unsigned long long K[T];
+  __shared__ uint32_t cnt[kBlock + 1];
+  const uint32_t tile = blockIdx.x, t = threadIdx.x;
+  uint32_t e0, e1, a0 = 0, g0 = 0;
+  if (tiles4) {
+    const uint4 a = tiles4[tile], b = tiles4[tile + 1];
+    e0 = a.y;
+    e1 = b.y;
+    a0 = a.z;
+    g0 = a.w;
+  } else {
+    e0 = tiles2[tile].y;
+    e1 = tiles2[tile + 1].y;
+  }
+  const uint32_t count = e1 - e0;
+  if (count > T) {  // one long segment: no dictionary
+    if (!tiles4 && t == 0) nd_out[tile] = 0;
+    return;
+  }
+  for (uint32_t i = t; i < T; i += kBlock)
+    K[i] = i < count ? (((unsigned long long)el[e0 + i].x << 16) | i) : ~0ull;
+  __syncthreads();
+  for (uint32_t k = 2; k <= T; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = t; i < T; i += kBlock) {
+        const uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          const unsigned long long a = K[i], b = K[ixj];
+          const bool up = (i & k) == 0;
+          if ((a > b) == up) {
+            K[i] = b;
+            K[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // thread t owns sorted entries [t*EPT, t*EPT + EPT)
+  const uint32_t q0 = t * EPT;
+  uint32_t mine = 0;
+  for (uint32_t q = q0; q < q0 + EPT && q < count; ++q)
+    mine += (q == 0 || (K[q] >> 16) != (K[q - 1] >> 16)) ? 1u : 0u;
+  cnt[t] = mine;
+  __syncthreads();
+  if (t == 0) {
+    uint32_t acc = 0;
+    for (int i = 0; i < kBlock; ++i) {
+      const uint32_t c = cnt[i];
+      cnt[i] = acc;
+      acc += c;
+    }
+    cnt[kBlock] = acc;
+  }
+  __syncthreads();
+  if (!tiles4) {
+    if (t == 0) nd_out[tile] = cnt[kBlock];
+    return;
+  }
+  uint32_t rank = cnt[t];
+  for (uint32_t q = q0; q < q0 + EPT && q < count; ++q) {
+    const uint32_t idx = (uint32_t)(K[q] >> 16), pos = (uint32_t)(K[q] & 0xFFFFu);
+    if (q == 0 || (K[q] >> 16) != (K[q - 1] >> 16)) dict[g0 + rank++] = idx;
+    dlix[a0 + pos] = (uint16_t)(rank - 1);
+    dval[a0 + pos] = __uint_as_float(el[e0 + pos].y);
+  }
+}
+
 // ------------------------------------------------------ atomic alternative --
 template <int G>
 __global__ __launch_bounds__(kBlock) void k_scatter(const uint32_t* __restrict__ rowptr,
@@ -276,64 +651,6 @@ __global__ __launch_bounds__(kBlock) void k_colsum_atomic(const uint2* __restric
     const u32x2 e = ld_stream(csr + k);
     unsafeAtomicAdd(&norm[e.x], __uint_as_float(e.y));
   }
-}
-
-__global__ __launch_bounds__(kBlock) void k_check_columns(const uint2* __restrict__ csr,
-                                                          uint64_t nnz, uint32_t cols,
-                                                          unsigned* __restrict__ bad) {
-  unsigned cnt = 0;
-  for (uint64_t k = blockIdx.x * (uint64_t)kBlock + threadIdx.x; k < nnz;
-       k += (uint64_t)gridDim.x * kBlock)
-    cnt += (csr[k].x >= cols) ? 1u : 0u;
-  if (cnt) atomicAdd(bad, cnt);
-}
-
-__global__ __launch_bounds__(kBlock) void k_rowptr(const uint64_t* __restrict__ rowidx,
-                                                   uint32_t rows, uint32_t* __restrict__ rowptr) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i == 0) rowptr[0] = 0;
-  if (i < rows) rowptr[i + 1] = (uint32_t)rowidx[i];
-}
-
-// CSC keys/values: key = column, value = {row, value bits} as one u64 (the
-// CSC record layout), one G=16 lane group per row.
-__global__ __launch_bounds__(kBlock) void k_csc_keys(const uint32_t* __restrict__ rowptr,
-                                                     const uint2* __restrict__ csr, uint32_t rows,
-                                                     uint32_t* __restrict__ keys,
-                                                     unsigned long long* __restrict__ vals) {
-  constexpr int G = 16;
-  const uint64_t t = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
-  const uint64_t row = t / G;
-  const uint32_t lane = (uint32_t)(t % G);
-  if (row >= rows) return;
-  for (uint32_t k = rowptr[row] + lane; k < rowptr[row + 1]; k += G) {
-    const uint2 e = csr[k];
-    keys[k] = e.x;
-    vals[k] = ((unsigned long long)e.y << 32) | (unsigned long long)row;
-  }
-}
-
-// colptr from the column-sorted keys: colptr[c] = first position with key >= c.
-__global__ __launch_bounds__(kBlock) void k_colptr(const uint32_t* __restrict__ skeys, uint64_t nnz,
-                                                   uint32_t cols, uint32_t* __restrict__ colptr) {
-  const uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
-  if (i >= nnz) return;
-  const uint32_t c = skeys[i];
-  if (i == 0) {
-    for (uint32_t cc = 0; cc <= c; ++cc) colptr[cc] = 0u;
-  } else {
-    const uint32_t p = skeys[i - 1];
-    if (p != c)
-      for (uint32_t cc = p + 1; cc <= c; ++cc) colptr[cc] = (uint32_t)i;
-  }
-  if (i == nnz - 1)
-    for (uint64_t cc = (uint64_t)c + 1; cc <= cols; ++cc) colptr[cc] = (uint32_t)nnz;
-}
-
-__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
-  for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * kBlock)
-    p[i] = v;
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill(float* p, float v, uint64_t n) {
@@ -417,14 +734,223 @@ void scatter_g(const DevMatrix& m, const float* r, float* u, hipStream_t s) {
                      m.rows, g.chunk);
 }
 
+uint32_t tile_grid(uint32_t ntiles) { return (std::max<uint32_t>(ntiles, 1) + 7) / 8 * 8; }
+
+template <int EPT>
+void fwd_t(const DevMatrix& m, const float* x, const int32_t* y, float* out, bool ratio,
+           const StatsState* st, hipStream_t s) {
+  if (m.tiles_fwd_ept != EPT || !m.tiles_fwd) raise(MLEM_ESTATE, "forward tiles not prepared");
+  const bool stats = ratio && st != nullptr;
+  const dim3 grid(tile_grid(m.ntiles_fwd)), block(kBlock);
+  if (stats && grid.x > st->max_blocks) raise(MLEM_ESTATE, "statistics buffer too small for launch");
+  if (stats)
+    hipLaunchKernelGGL((k_fwd_tiles<EPT, true, true>), grid, block, 0, s, m.rowptr, m.csr,
+                       m.tiles_fwd, m.ntiles_fwd, x, y, out, st->partials, st->counter,
+                       st->results, st->ring);
+  else if (ratio)
+    hipLaunchKernelGGL((k_fwd_tiles<EPT, true, false>), grid, block, 0, s, m.rowptr, m.csr,
+                       m.tiles_fwd, m.ntiles_fwd, x, y, out, nullptr, nullptr, nullptr, 0u);
+  else
+    hipLaunchKernelGGL((k_fwd_tiles<EPT, false, false>), grid, block, 0, s, m.rowptr, m.csr,
+                       m.tiles_fwd, m.ntiles_fwd, x, y, out, nullptr, nullptr, nullptr, 0u);
+}
+
+template <int EPT>
+void bwd_t(const DevMatrix& m, const float* r, const float* norm, float* out, bool update,
+           hipStream_t s) {
+  if (m.tiles_bwd_ept != EPT || !m.tiles_bwd) raise(MLEM_ESTATE, "backward tiles not prepared");
+  const dim3 grid(tile_grid(m.ntiles_bwd)), block(kBlock);
+  if (update)
+    hipLaunchKernelGGL((k_bwd_tiles<EPT, true>), grid, block, 0, s, m.colptr, m.csc, m.tiles_bwd,
+                       m.ntiles_bwd, r, norm, out);
+  else
+    hipLaunchKernelGGL((k_bwd_tiles<EPT, false>), grid, block, 0, s, m.colptr, m.csc, m.tiles_bwd,
+                       m.ntiles_bwd, r, norm, out);
+}
+
 void check_launch(const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) raise(MLEM_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+template <int EPT>
+void fwd_d(const DevMatrix& m, const float* x, const int32_t* y, float* out, bool ratio,
+           const StatsState* st, hipStream_t s) {
+  const DictFormat& D = m.dfwd;
+  if (D.ept != EPT || !D.tiles) raise(MLEM_ESTATE, "forward dictionary not prepared");
+  const bool stats = ratio && st != nullptr;
+  const dim3 grid(tile_grid(D.ntiles)), block(kBlock);
+  if (stats && grid.x > st->max_blocks) raise(MLEM_ESTATE, "statistics buffer too small for launch");
+  if (stats)
+    hipLaunchKernelGGL((k_fwd_dict<EPT, true, true>), grid, block, 0, s, m.rowptr, m.csr, D.tiles,
+                       D.ntiles, D.dval, D.dlix, D.dict, x, y, out, st->partials, st->counter,
+                       st->results, st->ring);
+  else if (ratio)
+    hipLaunchKernelGGL((k_fwd_dict<EPT, true, false>), grid, block, 0, s, m.rowptr, m.csr, D.tiles,
+                       D.ntiles, D.dval, D.dlix, D.dict, x, y, out, nullptr, nullptr, nullptr, 0u);
+  else
+    hipLaunchKernelGGL((k_fwd_dict<EPT, false, false>), grid, block, 0, s, m.rowptr, m.csr,
+                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, x, y, out, nullptr, nullptr,
+                       nullptr, 0u);
+}
+
+template <int EPT>
+void bwd_d(const DevMatrix& m, const float* r, const float* norm, float* out, bool update,
+           hipStream_t s) {
+  const DictFormat& D = m.dbwd;
+  if (D.ept != EPT || !D.tiles) raise(MLEM_ESTATE, "backward dictionary not prepared");
+  const dim3 grid(tile_grid(D.ntiles)), block(kBlock);
+  if (update)
+    hipLaunchKernelGGL((k_bwd_dict<EPT, true>), grid, block, 0, s, m.colptr, m.csc, D.tiles,
+                       D.ntiles, D.dval, D.dlix, D.dict, r, norm, out);
+  else
+    hipLaunchKernelGGL((k_bwd_dict<EPT, false>), grid, block, 0, s, m.colptr, m.csc, D.tiles,
+                       D.ntiles, D.dval, D.dlix, D.dict, r, norm, out);
+}
+
+// Greedy partition of segments [0, n) into tiles of at most T elements and
+// 256 segments (one per lane); a segment longer than T forms a tile of its own.
+std::vector<uint32_t> partition_tiles(const std::vector<uint32_t>& ptr, uint32_t T) {
+  const uint32_t n = (uint32_t)ptr.size() - 1;
+  std::vector<uint32_t> tiles{0};
+  uint32_t s = 0;
+  while (s < n) {
+    uint32_t e = s;
+    if (ptr[s + 1] - ptr[s] > T) {
+      e = s + 1;
+    } else {
+      uint64_t cnt = 0;
+      while (e < n && e - s < (uint32_t)kBlock && cnt + (ptr[e + 1] - ptr[e]) <= T) {
+        cnt += ptr[e + 1] - ptr[e];
+        ++e;
+      }
+    }
+    tiles.push_back(e);
+    s = e;
+  }
+  return tiles;
+}
+
+// Tile descriptors {first segment, first element}, ntiles + 1 entries.
+void upload_tiles(const std::vector<uint32_t>& ptr, int ept, uint2*& dev, uint32_t& ntiles,
+                  int& cur, hipStream_t s) {
+  if (cur == ept && dev) return;
+  const std::vector<uint32_t> t = partition_tiles(ptr, (uint32_t)(kBlock * ept));
+  std::vector<uint2> d(t.size());
+  for (size_t i = 0; i < t.size(); ++i) d[i] = make_uint2(t[i], ptr[t[i]]);
+  if (dev) (void)hipFree(dev);
+  dev = nullptr;
+  cur = 0;
+  MLEM_HIP(hipMalloc(&dev, sizeof(uint2) * d.size()));
+  MLEM_HIP(hipMemcpyAsync(dev, d.data(), sizeof(uint2) * d.size(), hipMemcpyHostToDevice, s));
+  MLEM_HIP(hipStreamSynchronize(s));
+  ntiles = (uint32_t)t.size() - 1;
+  cur = ept;
+}
+
+// Build the dictionary format of one projection from its device-order
+// segments (host copy of the pointers, device 8-byte records).
+template <int EPT>
+void build_dict(const std::vector<uint32_t>& ptr, const uint2* el, DictFormat& D, hipStream_t s) {
+  constexpr uint32_t T = kBlock * EPT;
+  const std::vector<uint32_t> tl = partition_tiles(ptr, T);
+  const uint32_t nt = (uint32_t)tl.size() - 1;
+  std::vector<uint2> d2(tl.size());
+  for (size_t i = 0; i < tl.size(); ++i) d2[i] = make_uint2(tl[i], ptr[tl[i]]);
+  struct Buf {
+    void* p = nullptr;
+    ~Buf() {
+      if (p) (void)hipFree(p);
+    }
+  } t2, ndb;
+  MLEM_HIP(hipMalloc(&t2.p, sizeof(uint2) * d2.size()));
+  MLEM_HIP(hipMalloc(&ndb.p, 4ull * std::max<uint32_t>(nt, 1)));
+  MLEM_HIP(hipMemcpyAsync(t2.p, d2.data(), sizeof(uint2) * d2.size(), hipMemcpyHostToDevice, s));
+  std::vector<uint32_t> nd(nt);
+  if (nt) {
+    hipLaunchKernelGGL((k_dict_build<EPT>), dim3(nt), dim3(kBlock), 0, s, (const uint2*)el,
+                       (const uint2*)t2.p, (const uint4*)nullptr, (uint32_t*)ndb.p,
+                       (uint32_t*)nullptr, (float*)nullptr, (uint16_t*)nullptr);
+    check_launch("dictionary count launch");
+    MLEM_HIP(hipMemcpyAsync(nd.data(), ndb.p, 4ull * nt, hipMemcpyDeviceToHost, s));
+    MLEM_HIP(hipStreamSynchronize(s));
+  }
+  std::vector<uint4> d4(tl.size());
+  uint64_t acc_a = 0, acc_g = 0;
+  for (uint32_t i = 0; i <= nt; ++i) {
+    d4[i] = make_uint4(tl[i], ptr[tl[i]], (uint32_t)acc_a, (uint32_t)acc_g);
+    if (i == nt) break;
+    const uint32_t count = ptr[tl[i + 1]] - ptr[tl[i]];
+    if (count <= T) acc_a += (count + 63) / 64 * 64;
+    acc_g += nd[i];
+  }
+  if (acc_a + T > 0xFFFFFFFFull || acc_g > 0xFFFFFFFFull)
+    raise(MLEM_EINVAL, "matrix too large for the dictionary format");
+  free_dict(D);
+  D.ntiles = nt;
+  D.ndict = std::max<uint64_t>(acc_g, 1);
+  D.nval = acc_a + T;  // tail: full-tile loads of the last tile stay in bounds
+  MLEM_HIP(hipMalloc(&D.tiles, sizeof(uint4) * d4.size()));
+  MLEM_HIP(hipMalloc(&D.dict, 4ull * D.ndict));
+  MLEM_HIP(hipMalloc(&D.dval, 4ull * D.nval));
+  MLEM_HIP(hipMalloc(&D.dlix, 2ull * D.nval));
+  MLEM_HIP(hipMemsetAsync(D.dval, 0, 4ull * D.nval, s));
+  MLEM_HIP(hipMemsetAsync(D.dlix, 0, 2ull * D.nval, s));
+  MLEM_HIP(hipMemcpyAsync(D.tiles, d4.data(), sizeof(uint4) * d4.size(), hipMemcpyHostToDevice, s));
+  if (nt) {
+    hipLaunchKernelGGL((k_dict_build<EPT>), dim3(nt), dim3(kBlock), 0, s, (const uint2*)el,
+                       (const uint2*)nullptr, (const uint4*)D.tiles, (uint32_t*)nullptr, D.dict,
+                       D.dval, D.dlix);
+    check_launch("dictionary fill launch");
+  }
+  MLEM_HIP(hipStreamSynchronize(s));
+  D.ept = EPT;
+}
+
+void ensure_dict(const std::vector<uint32_t>& ptr, const uint2* el, int ept, DictFormat& D,
+                 hipStream_t s) {
+  if (D.ept == ept && D.tiles) return;
+  switch (ept) {
+    case 8: build_dict<8>(ptr, el, D, s); break;
+    case 16: build_dict<16>(ptr, el, D, s); break;
+    default: raise(MLEM_EINVAL, "unsupported dictionary tile size");
+  }
+}
+
 }  // namespace
 
+void free_dict(DictFormat& d) {
+  if (d.tiles) (void)hipFree(d.tiles);
+  if (d.dict) (void)hipFree(d.dict);
+  if (d.dval) (void)hipFree(d.dval);
+  if (d.dlix) (void)hipFree(d.dlix);
+  d = DictFormat();
+}
+
+void prepare_tiles(DevMatrix& m, hipStream_t s) {
+  if (m.h_rowptr.size() != (size_t)m.rows + 1) {
+    m.h_rowptr.resize((size_t)m.rows + 1);
+    MLEM_HIP(hipMemcpyAsync(m.h_rowptr.data(), m.rowptr, 4ull * (m.rows + 1ull),
+                            hipMemcpyDeviceToHost, s));
+    MLEM_HIP(hipStreamSynchronize(s));
+  }
+  if (m.csc && m.h_colptr.size() != (size_t)m.cols + 1) {
+    m.h_colptr.resize((size_t)m.cols + 1);
+    MLEM_HIP(hipMemcpyAsync(m.h_colptr.data(), m.colptr, 4ull * (m.cols + 1ull),
+                            hipMemcpyDeviceToHost, s));
+    MLEM_HIP(hipStreamSynchronize(s));
+  }
+  if (m.fwd.kind == kTiles)
+    upload_tiles(m.h_rowptr, m.fwd.ept, m.tiles_fwd, m.ntiles_fwd, m.tiles_fwd_ept, s);
+  if (m.bwd.kind == kTiles && m.csc)
+    upload_tiles(m.h_colptr, m.bwd.ept, m.tiles_bwd, m.ntiles_bwd, m.tiles_bwd_ept, s);
+  if (m.fwd.kind == kDict) ensure_dict(m.h_rowptr, m.csr, m.fwd.ept, m.dfwd, s);
+  if (m.bwd.kind == kDict && m.csc) ensure_dict(m.h_colptr, m.csc, m.bwd.ept, m.dbwd, s);
+}
+
 unsigned stats_max_blocks(const DevMatrix& m) {
+  if (m.fwd.kind == kDict) return tile_grid(m.dfwd.ntiles);
+  if (m.fwd.kind == kTiles) return tile_grid(m.ntiles_fwd);
   const Geometry g = geometry(m.rows, m.fwd.lanes, m.fwd.blocks_per_cu, m.cus, true);
   return g.grid;
 }
@@ -433,6 +959,19 @@ void launch_fwd(const DevMatrix& m, int mode, const float* x, const int32_t* y, 
                 bool ratio, const StatsState* stats, hipStream_t s) {
   if (mode == MLEM_MODE_EXACT) {
     fwd_g<1, true>(m, x, y, out, ratio, stats, s);
+  } else if (m.fwd.kind == kDict) {
+    switch (m.fwd.ept) {
+      case 8: fwd_d<8>(m, x, y, out, ratio, stats, s); break;
+      case 16: fwd_d<16>(m, x, y, out, ratio, stats, s); break;
+      default: raise(MLEM_EINVAL, "unsupported forward dictionary tile size");
+    }
+  } else if (m.fwd.kind == kTiles) {
+    switch (m.fwd.ept) {
+      case 4: fwd_t<4>(m, x, y, out, ratio, stats, s); break;
+      case 8: fwd_t<8>(m, x, y, out, ratio, stats, s); break;
+      case 16: fwd_t<16>(m, x, y, out, ratio, stats, s); break;
+      default: raise(MLEM_EINVAL, "unsupported forward tile size");
+    }
   } else {
     switch (m.fwd.lanes) {
       case 4: fwd_g<4, false>(m, x, y, out, ratio, stats, s); break;
@@ -451,6 +990,19 @@ void launch_bwd(const DevMatrix& m, int mode, const float* r, const float* norm,
   if (!m.csc) raise(MLEM_ESTATE, "matrix was opened without a CSC transpose");
   if (mode == MLEM_MODE_EXACT) {
     bwd_g<1, true>(m, r, norm, out, update, s);
+  } else if (m.bwd.kind == kDict) {
+    switch (m.bwd.ept) {
+      case 8: bwd_d<8>(m, r, norm, out, update, s); break;
+      case 16: bwd_d<16>(m, r, norm, out, update, s); break;
+      default: raise(MLEM_EINVAL, "unsupported backward dictionary tile size");
+    }
+  } else if (m.bwd.kind == kTiles) {
+    switch (m.bwd.ept) {
+      case 4: bwd_t<4>(m, r, norm, out, update, s); break;
+      case 8: bwd_t<8>(m, r, norm, out, update, s); break;
+      case 16: bwd_t<16>(m, r, norm, out, update, s); break;
+      default: raise(MLEM_EINVAL, "unsupported backward tile size");
+    }
   } else {
     switch (m.bwd.lanes) {
       case 4: bwd_g<4, false>(m, r, norm, out, update, s); break;
@@ -502,19 +1054,6 @@ void launch_colsums_atomic(const DevMatrix& m, float* norm, hipStream_t s) {
   check_launch("column sums launch");
 }
 
-void launch_check_columns(const DevMatrix& m, unsigned* bad, hipStream_t s) {
-  hipLaunchKernelGGL(k_check_columns, dim3(std::min<uint32_t>(blocks_for(m.nnz), 8 * m.cus)),
-                     dim3(kBlock), 0, s, m.csr, m.nnz, m.cols, bad);
-  check_launch("column check launch");
-}
-
-void launch_rowptr_from_rowidx(const uint64_t* rowidx, uint32_t rows, uint32_t* rowptr,
-                               hipStream_t s) {
-  hipLaunchKernelGGL(k_rowptr, dim3(blocks_for((uint64_t)rows + 1)), dim3(kBlock), 0, s, rowidx,
-                     rows, rowptr);
-  check_launch("row pointer launch");
-}
-
 void launch_fill(float* p, float v, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_fill, dim3(blocks_for(n)), dim3(kBlock), 0, s, p, v, (uint64_t)n);
   check_launch("fill launch");
@@ -523,50 +1062,6 @@ void launch_fill(float* p, float v, size_t n, hipStream_t s) {
 void launch_copy(const uint4* src, uint4* dst, size_t n16, hipStream_t s) {
   hipLaunchKernelGGL(k_copy, dim3(blocks_for(n16)), dim3(kBlock), 0, s, src, dst, (uint64_t)n16);
   check_launch("copy launch");
-}
-
-void build_csc(DevMatrix& m, hipStream_t s) {
-  const uint64_t nnz = m.nnz;
-  MLEM_HIP(hipMalloc(&m.colptr, sizeof(uint32_t) * ((size_t)m.cols + 1)));
-  MLEM_HIP(hipMalloc(&m.csc, sizeof(uint2) * std::max<uint64_t>(nnz, 1)));
-  if (nnz == 0) {
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for((uint64_t)m.cols + 1)), dim3(kBlock), 0, s,
-                       m.colptr, 0u, (uint64_t)m.cols + 1);
-    check_launch("colptr fill");
-    return;
-  }
-  uint32_t* keys = nullptr;
-  uint32_t* skeys = nullptr;
-  unsigned long long* vals = nullptr;
-  void* temp = nullptr;
-  struct Guard {
-    void** ptrs[4];
-    ~Guard() {
-      for (void** p : ptrs)
-        if (*p) (void)hipFree(*p);
-    }
-  } guard{{(void**)&keys, (void**)&skeys, (void**)&vals, &temp}};
-  MLEM_HIP(hipMalloc(&keys, sizeof(uint32_t) * nnz));
-  MLEM_HIP(hipMalloc(&skeys, sizeof(uint32_t) * nnz));
-  MLEM_HIP(hipMalloc(&vals, sizeof(unsigned long long) * nnz));
-  const uint64_t threads = (uint64_t)m.rows * 16;
-  hipLaunchKernelGGL(k_csc_keys, dim3((uint32_t)((threads + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                     s, m.rowptr, m.csr, m.rows, keys, vals);
-  check_launch("csc key launch");
-  const uint32_t maxcol = m.cols > 0 ? m.cols - 1 : 0;
-  const int end_bit = maxcol == 0 ? 1 : 32 - __builtin_clz(maxcol);
-  size_t temp_bytes = 0;
-  MLEM_HIP(rocprim::radix_sort_pairs(nullptr, temp_bytes, keys, skeys, vals,
-                                     reinterpret_cast<unsigned long long*>(m.csc), (size_t)nnz, 0,
-                                     end_bit, s));
-  MLEM_HIP(hipMalloc(&temp, std::max<size_t>(temp_bytes, 16)));
-  MLEM_HIP(rocprim::radix_sort_pairs(temp, temp_bytes, keys, skeys, vals,
-                                     reinterpret_cast<unsigned long long*>(m.csc), (size_t)nnz, 0,
-                                     end_bit, s));
-  hipLaunchKernelGGL(k_colptr, dim3((uint32_t)((nnz + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                     skeys, nnz, m.cols, m.colptr);
-  check_launch("colptr launch");
-  MLEM_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace mlem

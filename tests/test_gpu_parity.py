@@ -29,20 +29,22 @@ TOL_RECON = 1e-4
 
 
 def _open(m, **kw):
-    return M.Csr4Matrix.from_arrays(m.rows, m.cols, m.rowidx, m.elements, **kw)
+    return M.Csr4Matrix.from_arrays(m.rows, m.cols, m.rowidx, m.elements, grid=m.grid, **kw)
 
 
-@pytest.fixture(scope="module")
-def tiny(tiny_problem):
-    A = _open(tiny_problem.matrix)
+# "layout": rows and voxels renumbered on the device (the default);
+# "file": file order kept (MLEM_OPEN_KEEP_ORDER).
+@pytest.fixture(scope="module", params=["layout", "file"])
+def tiny(tiny_problem, request):
+    A = _open(tiny_problem.matrix, keep_order=(request.param == "file"))
     yield tiny_problem, A
     A.close()
 
 
-@pytest.fixture(scope="module")
-def small():
+@pytest.fixture(scope="module", params=["layout", "file"])
+def small(request):
     p = synth.make_problem("small")
-    A = _open(p.matrix)
+    A = _open(p.matrix, keep_order=(request.param == "file"))
     yield p, A
     A.close()
 
@@ -185,23 +187,81 @@ def test_edge_cases_empty_rows_and_columns():
     A.close()
 
 
-def test_ragged_row_lengths():
-    # lengths 1..2000 exercise every lane-group tail
+FAMILIES = [("tiles", 4), ("tiles", 8), ("tiles", 16), ("dict", 8), ("dict", 16),
+            ("groups", 4), ("groups", 16), ("groups", 64)]
+
+
+@pytest.fixture(scope="module")
+def ragged():
+    # row lengths 0..299 plus rows longer than every tile (5000 > 4096), and
+    # columns from 1 to ~300 elements: every lane-group tail, partial tile and
+    # long-segment path
     rng = np.random.default_rng(8)
-    cols = 5000
-    rows = []
-    for L in list(range(1, 300)) + [511, 512, 513, 1000, 2000]:
+    cols = 6000
+    rows = [[]]
+    for L in list(range(1, 300)) + [511, 512, 513, 1000, 2000, 2049, 4097, 5000]:
         c = np.sort(rng.choice(cols, L, replace=False))
         rows.append([(int(j), float(np.float32(rng.uniform(0.1, 1.7)))) for j in c])
     ends = np.cumsum([len(r) for r in rows]).astype(np.uint64)
     el = np.array([e for r in rows for e in r], dtype=M.ELEMENT_DTYPE)
     A = M.Csr4Matrix.from_arrays(len(rows), cols, ends, el)
+    yield A, ends, el, cols, rng
+    A.close()
+
+
+def test_ragged_exact(ragged):
+    A, ends, el, cols, rng = ragged
     x = rng.uniform(0.1, 3, cols).astype(np.float32)
-    r = rng.uniform(0.1, 3, len(rows)).astype(np.float32)
+    r = rng.uniform(0.1, 3, A.rows()).astype(np.float32)
     assert np.array_equal(M.calcFwProj(A, x, "exact"), orc.fwproj(ends, el, x))
     assert np.array_equal(M.calcBkProj(A, r, "exact"), orc.bkproj(ends, el, r, cols))
+
+
+@pytest.mark.parametrize("family", FAMILIES, ids=lambda f: f"{f[0]}{f[1]}")
+def test_ragged_fast_families(ragged, family):
+    A, ends, el, cols, rng = ragged
+    A.set_launch(family, family)
+    assert A.launch() == {"fwd": family, "bwd": family}
+    x = rng.uniform(0.1, 3, cols).astype(np.float32)
+    r = rng.uniform(0.1, 3, A.rows()).astype(np.float32)
     assert rel_linf(M.calcFwProj(A, x, "fast"), orc.f64_fwproj(ends, el, x)) < TOL_PROJ
     assert rel_linf(M.calcBkProj(A, r, "fast"), orc.f64_bkproj(ends, el, r, cols)) < TOL_PROJ
+    y = rng.integers(0, 50, A.rows()).astype(np.int32)
+    xg, ll, _ = M.mlem(A, y, None, 6, mode="fast", stats=True)
+    xo = orc.mlem(ends, el, cols, y, 6)[0]
+    assert rel_linf(xg, xo) < 1e-5
+    _, ll64, _ = orc.f64_mlem(ends, el, cols, y.astype(np.float64), 6)
+    assert rel_linf(ll, ll64) < 1e-6
+
+
+@pytest.mark.parametrize("family", FAMILIES, ids=lambda f: f"{f[0]}{f[1]}")
+def test_small_fast_families(small, family):
+    # every kernel family on the 32^3 workload, 20 iterations
+    p, A = small
+    m = p.matrix
+    A.set_launch(family, family)
+    x = M.mlem(A, p.y, None, 20, mode="fast")
+    xo = orc.mlem(m.rowidx, m.elements, m.cols, p.y, 20)[0]
+    assert rel_linf(x, xo) < TOL_RECON
+    A.set_launch(("tiles", 8), ("tiles", 8))
+
+
+def test_layout_with_partial_bricks():
+    # 7x5x3 grid: bricks of 4x4x2 are cut at every face
+    m = synth.make_matrix((7, 5, 3), 3000, 4242)
+    rng = np.random.default_rng(9)
+    y = rng.integers(0, 30, m.rows).astype(np.int32)
+    A = _open(m)
+    x = rng.uniform(0.1, 3, m.cols).astype(np.float32)
+    r = rng.uniform(0.1, 3, m.rows).astype(np.float32)
+    assert np.array_equal(M.calcFwProj(A, x, "exact"), orc.fwproj(m.rowidx, m.elements, x))
+    assert np.array_equal(M.calcBkProj(A, r, "exact"), orc.bkproj(m.rowidx, m.elements, r, m.cols))
+    assert np.array_equal(M.calcColumnSums(A), orc.column_sums(m.rowidx, m.elements, m.cols))
+    assert np.array_equal(M.mlem(A, y, None, 8, mode="exact"),
+                          orc.mlem(m.rowidx, m.elements, m.cols, y, 8)[0])
+    x0 = rng.uniform(0.5, 1.5, m.cols).astype(np.float32)
+    assert np.array_equal(M.mlem(A, y, x0.copy(), 4, mode="exact", x0_given=True),
+                          orc.mlem(m.rowidx, m.elements, m.cols, y, 4, x0=x0)[0])
     A.close()
 
 
