@@ -127,11 +127,14 @@ def main() -> int:
         name = {"fwd": "k_fwd (CSR forward projection + fused ratio)",
                 "bwd": "k_scatter + k_update (atomic backprojection)"}[dom]
 
+    # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    # FETCH_SIZE/WRITE_SIZE passes (copy-kernel calibrated) of this workload
+    # and launch configuration.
     traffic = None
     tfile = ROOT / "profiles" / f"traffic_{args.workload}_{args.mode}.json"
     if tfile.exists():
         t = json.loads(tfile.read_text())
-        if t.get("launch") == launch:
+        if t.get("launch") == json.loads(json.dumps(launch)):
             traffic = t.get(dom)
 
     # parity sample + CPU baseline (same iteration count)

@@ -362,8 +362,8 @@ void finish_open(mlem_matrix* h, unsigned flags) {
   h->norm_seconds = seconds_since(t0);
 
   const double avgRow = m.rows ? (double)m.nnz / m.rows : 1.0;
-  m.fwd = make_launch(kTiles, 8);
-  m.bwd = make_launch(kTiles, 8);
+  m.fwd = make_launch(kDict, 16);
+  m.bwd = make_launch(kDict, 16);
   m.scatter = make_launch(kGroups, lanes_for(avgRow));
   prepare_tiles(m, m.stream);
   if (env_int("MLEM_AUTOTUNE", 0) > 0) autotune(m);
