@@ -57,6 +57,51 @@ def cpu_baseline(problem, iters: int, parity_x: np.ndarray | None):
     return out, parity
 
 
+def batched_run(A, prob, frames: int, width: int, plain_value: float) -> dict:
+    """Dynamic-frame workload on the same matrix: per-frame phantom scaling
+    U(0.2, 2) per ellipsoid, total counts log-uniform in [1e5, 1e8], iteration
+    budgets uniform in 10..100, early stop at a relative log-likelihood change
+    below 1e-6. Returns frame-iterations/s and the parity of the frame with the
+    smallest budget against the fp32 sequential restatement."""
+    import mlem_amd as M
+    from mlem_amd import synth
+    w = synth.WORKLOADS[prob.name]
+    t0 = time.time()
+    counts = synth.frame_counts(frames)
+    budgets = synth.frame_budgets(frames)
+    Y = np.empty((frames, prob.matrix.rows), np.int32)
+    for f, sc in enumerate(synth.frame_scales(frames, w["ellipsoids"])):
+        x = synth.make_phantom(w["grid"], w["ellipsoids"], 2000 + w["key"], w["semi"], scale=sc)
+        Y[f] = synth.poisson(synth.project(prob.matrix, x), counts[f], 5000 + f)
+    gen = time.time() - t0
+    B = M.Batch(A, Y, budgets, rel_tol=1e-6, block_width=width)
+    B.run(1)  # warm-up: graph capture, first launches
+    done0 = B.progress()["frame_iterations"]
+    M.device_sync()
+    t0 = time.perf_counter()
+    n = B.run(1000)
+    M.device_sync()
+    wall = time.perf_counter() - t0
+    fi = B.progress()["frame_iterations"] - done0
+    its = [B.frame_info(f)[0] for f in range(frames)]
+    out = {"frames": frames, "block_width": width, "frame_iterations": int(fi),
+           "batched_iterations": n + 1, "wall_s": wall,
+           "frame_iterations_per_s": fi / wall,
+           "speedup_vs_plain": (fi / wall) / plain_value,
+           "iterations_per_frame": {"min": int(min(its)), "max": int(max(its)),
+                                    "mean": float(np.mean(its))},
+           "early_stopped_frames": int(sum(1 for f in range(frames) if its[f] < budgets[f])),
+           "generation_s": gen}
+    f = int(np.argmin(its))
+    from oracle import oracle as orc  # parity check only
+    m = prob.matrix
+    xo = orc.mlem(m.rowidx, m.elements, m.cols, Y[f], its[f])[0]
+    out["parity"] = {"frame": f, "iterations": its[f],
+                     "bit_identical_to_seq_fp32": bool(np.array_equal(B.image(f), xo))}
+    B.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +115,9 @@ def main() -> int:
     ap.add_argument("--keep-order", action="store_true",
                     help="keep file row/voxel order on the device (no locality layout)")
     ap.add_argument("--launch", default="", help="family,param,family,param e.g. tiles,8,groups,32")
+    ap.add_argument("--batched-frames", type=int, default=64,
+                    help="frames of the batched (dynamic-frame) measurement; 0 = skip")
+    ap.add_argument("--batched-width", type=int, default=32)
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="first run the plain copy kernel over this many MiB (calibrates "
                          "hardware byte counters under rocprofv3 --pmc)")
@@ -148,6 +196,14 @@ def main() -> int:
         cpu, parity = cpu_baseline(prob, args.cpu_iters, xg)
         cpu["threads_available"] = os.cpu_count()
 
+    batched = None
+    if args.batched_frames > 0 and args.mode != "atomic":
+        try:
+            batched = batched_run(A, prob, args.batched_frames, args.batched_width, value)
+        except Exception as e:  # reported in the line, never hides the plain result
+            batched = {"error": f"{type(e).__name__}: {e}"}
+        print(f"[bench] batched: {batched}", file=sys.stderr, flush=True)
+
     line = {
         "metric": METRIC,
         "value": value,
@@ -183,6 +239,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "parity": parity,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
     R.close()

@@ -59,6 +59,13 @@ extern "C" {
 #define MLEM_OPEN_NO_CSC 0x1u /* skip the CSC build (ATOMIC mode only)   */
 #define MLEM_OPEN_KEEP_ORDER 0x2u /* keep file row/voxel order on the device
                                      (no locality layout)                   */
+/* Reduced-precision storage of the matrix values: every row is scaled by its
+ * largest value (an fp32 per-row scale s_i) and the quotients q_ij are stored
+ * in fp16 or bf16; column indices stay exact. Projections accumulate in fp32;
+ * the column sums n_j = sum_i s_i q_ij and the log-likelihood run in FP64.
+ * MLEM only sees a_ij = s_i q_ij. */
+#define MLEM_OPEN_FP16 0x4u
+#define MLEM_OPEN_BF16 0x8u
 
 typedef struct mlem_matrix mlem_matrix; /* device-resident system matrix */
 typedef struct mlem_recon mlem_recon;   /* one reconstruction's state    */
@@ -204,6 +211,41 @@ MLEM_API int mlem_recon_run_timed(mlem_recon* r, int iterations, float* kernel_f
 MLEM_API int mlem_recon_image(mlem_recon* r, float* x, uint32_t cols);
 /* Iterations completed so far. */
 MLEM_API int mlem_recon_iterations(const mlem_recon* r);
+
+/* ---- batched reconstruction: many sinograms, one matrix ------------------
+ * Frames (e.g. dynamic PET frames) are reconstructed together in blocks of up
+ * to block_width right-hand sides; each matrix record streamed from HBM
+ * serves a whole block. Per frame the arithmetic is that of mlem()
+ * (seqcsr4mlem.cpp:137-189): the image after k iterations is bit-identical to
+ * k iterations of the sequential driver. A frame stops after budgets[f]
+ * iterations or, with rel_tol > 0, after iteration k >= 2 once
+ * |L(x_{k-1}) - L(x_{k-2})| <= rel_tol * |L(x_{k-1})|, L being the FP64
+ * Poisson log-likelihood sum_{f_i != 0} (y_i log f_i - f_i) that iteration k
+ * computes for the image entering it. Finished frames are frozen and the rest
+ * repacked into fewer / narrower blocks. No reference counterpart (the
+ * reference reconstructs one sinogram per run). */
+typedef struct mlem_batch mlem_batch;
+typedef struct mlem_batch_options {
+  double rel_tol;   /* early-stop tolerance on the log-likelihood (0 = off) */
+  int block_width;  /* widest RHS block: 4, 8, 16 or 32 frames            */
+  int use_graph;    /* 1: replay each iteration from a captured hipGraph  */
+} mlem_batch_options;
+MLEM_API void mlem_batch_options_default(mlem_batch_options* o);
+/* y: frames*rows int32 counts, frame-major; budgets: frames iteration limits
+ * (NULL: unlimited). x0 of each frame follows initImage. */
+MLEM_API int mlem_batch_create(mlem_matrix* m, uint32_t frames, const int32_t* y,
+                               uint32_t rows, const int32_t* budgets,
+                               const mlem_batch_options* o, mlem_batch** out);
+/* Run up to max_iterations batched iterations (fewer if every frame stops). */
+MLEM_API int mlem_batch_run(mlem_batch* b, int max_iterations, int* iterations_run);
+MLEM_API int mlem_batch_image(mlem_batch* b, uint32_t frame, float* x, uint32_t cols);
+/* Iterations done by a frame and its log-likelihood history (capacity entries). */
+MLEM_API int mlem_batch_frame_info(mlem_batch* b, uint32_t frame, int* iterations,
+                                   double* loglik, int capacity);
+MLEM_API int mlem_batch_progress(const mlem_batch* b, uint32_t* active_frames,
+                                 uint64_t* frame_iterations, int* block_width,
+                                 uint32_t* blocks);
+MLEM_API void mlem_batch_destroy(mlem_batch* b);
 
 /* ---- launch configuration (autotuner) --------------------------------------
  * FAST-mode kernel family and its parameter, per projection:

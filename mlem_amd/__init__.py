@@ -25,7 +25,7 @@ from typing import Optional
 import numpy as np
 
 from ._native import (INIT_GIVEN, INIT_REFERENCE, MODE_ATOMIC, MODE_EXACT, MODE_FAST, MODES,
-                      OPEN_KEEP_ORDER, OPEN_NO_CSC, MlemError, Options, check, lib)
+                      OPEN_BF16, OPEN_FP16, OPEN_KEEP_ORDER, OPEN_NO_CSC, MlemError, Options, check, lib)
 
 __all__ = [
     "Csr4Matrix", "Recon", "MlemError", "ELEMENT_DTYPE", "probe", "calcColumnSums", "initImage",
@@ -55,8 +55,14 @@ def _i32(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.int32)
 
 
-def _flags(no_csc: bool, keep_order: bool) -> int:
-    return (OPEN_NO_CSC if no_csc else 0) | (OPEN_KEEP_ORDER if keep_order else 0)
+PRECISIONS = {"fp32": 0, "fp16": OPEN_FP16, "bf16": OPEN_BF16}
+
+
+def _flags(no_csc: bool, keep_order: bool, precision: str = "fp32") -> int:
+    if precision not in PRECISIONS:
+        raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+    return ((OPEN_NO_CSC if no_csc else 0) | (OPEN_KEEP_ORDER if keep_order else 0)
+            | PRECISIONS[precision])
 
 
 def probe(path: str) -> dict:
@@ -74,18 +80,23 @@ class Csr4Matrix:
 
     Csr4Matrix(path)                            load a csr4 file into HBM
     Csr4Matrix.from_arrays(rows, cols, rowidx, elements)   from host memory
+
+    precision "fp16" / "bf16" stores every row as an fp32 scale (the row's
+    largest value) times 16-bit quotients; "fp32" keeps the file's values.
     """
 
     def __init__(self, path: Optional[str] = None, *, no_csc: bool = False,
-                 keep_order: bool = False, _handle=None):
+                 keep_order: bool = False, precision: str = "fp32", _handle=None):
         self._h = None
+        self.precision = precision
         if _handle is not None:
             self._h = _handle
         else:
             if path is None:
                 raise ValueError("path required")
             h = C.c_void_p()
-            check(lib().mlem_open(os.fsencode(path), _flags(no_csc, keep_order), C.byref(h)))
+            check(lib().mlem_open(os.fsencode(path), _flags(no_csc, keep_order, precision),
+                                  C.byref(h)))
             self._h = h
         r, c, n = C.c_uint32(), C.c_uint32(), C.c_uint64()
         check(lib().mlem_dims(self._h, C.byref(r), C.byref(c), C.byref(n)))
@@ -93,8 +104,10 @@ class Csr4Matrix:
 
     @classmethod
     def from_arrays(cls, rows: int, cols: int, rowidx: np.ndarray, elements: np.ndarray,
-                    grid=None, no_csc: bool = False, keep_order: bool = False) -> "Csr4Matrix":
+                    grid=None, no_csc: bool = False, keep_order: bool = False,
+                    precision: str = "fp32") -> "Csr4Matrix":
         """grid = (nx, ny, nz) enables the device locality layout."""
+        flags = _flags(no_csc, keep_order, precision)
         rowidx = np.ascontiguousarray(rowidx, dtype=np.uint64)
         if elements.dtype != ELEMENT_DTYPE:
             elements = np.ascontiguousarray(elements).view(ELEMENT_DTYPE)
@@ -105,8 +118,8 @@ class Csr4Matrix:
         g = None if grid is None else (C.c_uint32 * 3)(*grid)
         h = C.c_void_p()
         check(lib().mlem_open_host(rows, cols, nnz, _ptr(rowidx), _ptr(elements), g,
-                                   _flags(no_csc, keep_order), C.byref(h)))
-        return cls(_handle=h)
+                                   flags, C.byref(h)))
+        return cls(_handle=h, precision=precision)
 
     def rows(self) -> int:
         return self._rows
@@ -287,6 +300,67 @@ class Recon:
     def close(self) -> None:
         if getattr(self, "_h", None) is not None:
             lib().mlem_recon_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many sinograms reconstructed against one matrix (RHS blocks).
+
+    y: int32 [frames, rows]; budgets: int [frames] or None; rel_tol > 0 stops a
+    frame on a small relative log-likelihood change.
+    """
+
+    def __init__(self, matrix: Csr4Matrix, y, budgets=None, *, rel_tol=0.0, block_width=32,
+                 use_graph=True):
+        from ._native import BatchOptions
+        self.matrix = matrix
+        y = np.ascontiguousarray(y, dtype=np.int32)
+        if y.ndim != 2 or y.shape[1] != matrix.rows():
+            raise ValueError("y must be [frames, rows]")
+        self.frames = y.shape[0]
+        b = None if budgets is None else np.ascontiguousarray(budgets, dtype=np.int32)
+        o = BatchOptions()
+        lib().mlem_batch_options_default(C.byref(o))
+        o.rel_tol = float(rel_tol)
+        o.block_width = int(block_width)
+        o.use_graph = 1 if use_graph else 0
+        h = C.c_void_p()
+        check(lib().mlem_batch_create(matrix.handle, self.frames, _ptr(y), y.shape[1], _ptr(b),
+                                      C.byref(o), C.byref(h)))
+        self._h = h
+
+    def run(self, max_iterations: int) -> int:
+        n = C.c_int()
+        check(lib().mlem_batch_run(self._h, int(max_iterations), C.byref(n)))
+        return n.value
+
+    def image(self, frame: int) -> np.ndarray:
+        x = np.empty(self.matrix.columns(), np.float32)
+        check(lib().mlem_batch_image(self._h, int(frame), _ptr(x), x.size))
+        return x
+
+    def frame_info(self, frame: int, capacity: int = 4096):
+        it = C.c_int()
+        ll = np.zeros(capacity, np.float64)
+        check(lib().mlem_batch_frame_info(self._h, int(frame), C.byref(it), _ptr(ll), capacity))
+        return it.value, ll[:min(it.value, capacity)]
+
+    def progress(self) -> dict:
+        a, fi, w, nb = C.c_uint32(), C.c_uint64(), C.c_int(), C.c_uint32()
+        check(lib().mlem_batch_progress(self._h, C.byref(a), C.byref(fi), C.byref(w),
+                                        C.byref(nb)))
+        return {"active": a.value, "frame_iterations": fi.value, "block_width": w.value,
+                "blocks": nb.value}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            lib().mlem_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -27,6 +27,8 @@ MODES = {"fast": MODE_FAST, "exact": MODE_EXACT, "atomic": MODE_ATOMIC}
 
 OPEN_NO_CSC = 0x1
 OPEN_KEEP_ORDER = 0x2
+OPEN_FP16 = 0x4
+OPEN_BF16 = 0x8
 INIT_REFERENCE = 0
 INIT_GIVEN = 1
 
@@ -49,6 +51,14 @@ class Options(C.Structure):
         ("use_graph", C.c_int),
         ("loglik", C.POINTER(C.c_double)),
         ("fwd_total", C.POINTER(C.c_double)),
+    ]
+
+
+class BatchOptions(C.Structure):
+    _fields_ = [
+        ("rel_tol", C.c_double),
+        ("block_width", C.c_int),
+        ("use_graph", C.c_int),
     ]
 
 
@@ -85,6 +95,14 @@ SIGNATURES = {
                                   C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "mlem_set_launch": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
     "mlem_autotune": (C.c_int, [_P]),
+    "mlem_batch_options_default": (None, [C.c_void_p]),
+    "mlem_batch_create": (C.c_int, [_P, _u32, _P, _u32, _P, _P, C.POINTER(_P)]),
+    "mlem_batch_run": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
+    "mlem_batch_image": (C.c_int, [_P, _u32, _P, _u32]),
+    "mlem_batch_frame_info": (C.c_int, [_P, _u32, C.POINTER(C.c_int), _P, C.c_int]),
+    "mlem_batch_progress": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
+                                      C.POINTER(C.c_int), C.POINTER(C.c_uint32)]),
+    "mlem_batch_destroy": (None, [_P]),
     "mlem_calibration_copy": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_float)]),
     "mlem_device_sync": (C.c_int, []),
 }

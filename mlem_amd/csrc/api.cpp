@@ -40,32 +40,15 @@ void hip_check(hipError_t e, const char* what) {
 }  // namespace mlem
 
 namespace {
-
 thread_local std::string g_error;
+}  // namespace
 
-template <class F>
-int guarded(F&& f) {
-  try {
-    f();
-    g_error.clear();
-    return MLEM_OK;
-  } catch (const ApiError& e) {
-    g_error = e.msg;
-    return e.code;
-  } catch (const Csr4Error& e) {
-    g_error = e.msg;
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    g_error = "host allocation failed";
-    return MLEM_ENOMEM;
-  } catch (const std::exception& e) {
-    g_error = e.what();
-    return MLEM_EINVAL;
-  } catch (...) {
-    g_error = "unknown error";
-    return MLEM_EINVAL;
-  }
-}
+namespace mlem {
+void set_error(const std::string& msg) { g_error = msg; }
+void clear_error() { g_error.clear(); }
+}  // namespace mlem
+
+namespace {
 
 int env_int(const char* name, int dflt) {
   const char* v = std::getenv(name);
@@ -144,11 +127,8 @@ void stream_to_device(void* dst, uint64_t bytes, hipStream_t s, Produce&& produc
 }  // namespace
 
 // --------------------------------------------------------------- handles --
-struct mlem_matrix {
-  DevMatrix m;
-  double norm_seconds = 0.0;  // CSC build + column sums at open
-  double load_seconds = 0.0;  // file/host -> HBM
-  ~mlem_matrix() {
+mlem_matrix::~mlem_matrix() {
+  {
     dfree(m.rowptr);
     dfree(m.csr);
     dfree(m.colptr);
@@ -164,13 +144,14 @@ struct mlem_matrix {
     dfree(m.colmap);
     dfree(m.rowold);
     dfree(m.rowpos);
+    dfree(m.rscale);
     dfree(m.tiles_fwd);
     dfree(m.tiles_bwd);
     free_dict(m.dfwd);
     free_dict(m.dbwd);
     if (m.stream) (void)hipStreamDestroy(m.stream);
   }
-};
+}
 
 struct mlem_recon {
   mlem_matrix* mat = nullptr;
@@ -326,6 +307,10 @@ void ensure_scratch(DevMatrix& m);
 void finish_open(mlem_matrix* h, unsigned flags) {
   DevMatrix& m = h->m;
   auto t0 = std::chrono::steady_clock::now();
+  if ((flags & MLEM_OPEN_FP16) && (flags & MLEM_OPEN_BF16))
+    raise(MLEM_EINVAL, "MLEM_OPEN_FP16 and MLEM_OPEN_BF16 are exclusive");
+  if ((flags & (MLEM_OPEN_FP16 | MLEM_OPEN_BF16)) && (flags & MLEM_OPEN_NO_CSC))
+    raise(MLEM_EINVAL, "16-bit values need the CSC (MLEM_OPEN_NO_CSC not allowed)");
   if (m.nnz > 0) {
     unsigned* bad = nullptr;
     MLEM_HIP(hipMalloc(&bad, sizeof(unsigned)));
@@ -342,13 +327,33 @@ void finish_open(mlem_matrix* h, unsigned flags) {
       raise(MLEM_EFORMAT, std::to_string(hbad) + " matrix elements have a column index >= " +
                               std::to_string(m.cols));
   }
+  const int vk = (flags & MLEM_OPEN_FP16) ? kF16 : (flags & MLEM_OPEN_BF16) ? kBF16 : kF32;
+  struct Scratch {
+    float* p = nullptr;
+    ~Scratch() {
+      if (p) (void)hipFree(p);
+    }
+  } rscale_file;
+  if (vk != kF32) {
+    // Row scales and 16-bit values are fixed before the layout is built, so
+    // the CSC and both dictionary formats inherit the quantized values.
+    MLEM_HIP(hipMalloc(&m.rscale, 4ull * std::max<uint32_t>(m.rows, 1)));
+    MLEM_HIP(hipMalloc(&rscale_file.p, 4ull * std::max<uint32_t>(m.rows, 1)));
+    launch_quantize(m, vk, rscale_file.p, m.stream);
+  }
   const bool reorder = !(flags & MLEM_OPEN_KEEP_ORDER);
   build_layout(m, reorder, reorder, !(flags & MLEM_OPEN_NO_CSC), m.stream);
   MLEM_HIP(hipMalloc(&m.norm, 4ull * std::max<uint32_t>(m.cols, 1)));
-  if (m.csc)
+  if (vk != kF32) {
+    if (m.rows) rows_to_device(m, (const float*)rscale_file.p, m.rscale, m.stream);
+    MLEM_HIP(hipStreamSynchronize(m.stream));
+    m.vk = vk;
+    launch_colsums_scaled(m, m.norm, m.stream);
+  } else if (m.csc) {
     launch_colsums_csc(m, m.norm, m.stream);
-  else
+  } else {
     launch_colsums_atomic(m, m.norm, m.stream);
+  }
   ensure_scratch(m);
   to_file_cols(m, m.norm, m.tx, m.stream);
   std::vector<float> norm(m.cols);
@@ -723,6 +728,7 @@ int mlem_bkproj(mlem_matrix* h, const float* r, uint32_t rows, float* u, uint32_
     ensure_scratch(m);
     h2d(m.tf, r, 4ull * rows, m.stream);
     rows_to_device(m, (const float*)m.tf, m.sr, m.stream);
+    launch_scale_rows(m, m.sr, m.stream);  // 16-bit values: A^T r = Q^T (s r)
     if (mode == MLEM_MODE_ATOMIC) {
       launch_fill(m.su, 0.f, cols, m.stream);
       launch_scatter(m, m.sr, m.su, m.stream);

@@ -1,0 +1,644 @@
+// batch.hip — many sinograms (frames) reconstructed against one system
+// matrix: the RHS-block (SpMM) form of MLEM on gfx950.
+//
+// Frames are grouped into blocks of W (4, 8, 16 or 32) right-hand sides. A
+// block's image, counts and ratios are stored interleaved per voxel / row
+// (X[col*W + f]), so the W lanes that own one voxel's frames gather a whole
+// contiguous W*4-byte run (one 128-byte line at W = 32) and every matrix
+// record streamed from HBM serves W frames. Each 256-lane block walks whole
+// tiles of rows (CSC columns), stages the tile's 8-byte records in LDS with
+// coalesced loads, and gives every row (column) to a group of W lanes, lane f
+// accumulating frame f sequentially in file element order with separate
+// multiply and add: per frame, the result is bit-identical to the reference's
+// sequential driver (mlem_seq/seqcsr4mlem.cpp:69-134).
+//
+// Per frame and iteration the forward pass also produces the FP64 Poisson
+// log-likelihood sum_{f_i != 0} (y_i log f_i - f_i) and sum_i f_i with a
+// deterministic two-level reduction; the host stops a frame when its budget
+// is spent or the relative log-likelihood change drops below the tolerance,
+// freezes it, and repacks the remaining frames into fewer / narrower blocks.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "internal.hpp"
+#include "mlem.h"
+
+using namespace mlem;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kTile = 2048;  // records per tile (16 KB of LDS)
+constexpr unsigned kRing = 1024;  // per-iteration statistics ring
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+inline uint32_t blocks_for(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, (n + kBlock - 1) / kBlock); }
+
+void check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) raise(MLEM_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Deterministic per-frame FP64 reduction of (ll, tot) over the grid: block
+// partials, then the last block to finish sums them in a fixed order.
+template <int W>
+__device__ void batch_stats_finish(double ll, double tot, double2* __restrict__ partials,
+                                   unsigned* __restrict__ counter, double2* __restrict__ results,
+                                   unsigned ring) {
+  constexpr int G = kBlock / W;
+  __shared__ double s_ll[kBlock], s_tot[kBlock];
+  __shared__ int s_last;
+  const uint32_t t = threadIdx.x, f = t % W, g = t / W;
+  s_ll[t] = ll;
+  s_tot[t] = tot;
+  __syncthreads();
+  if (t < (uint32_t)W) {
+    double a = 0.0, b = 0.0;
+    for (int k = 0; k < G; ++k) {
+      a += s_ll[k * W + f];
+      b += s_tot[k * W + f];
+    }
+    __hip_atomic_store(&partials[(size_t)blockIdx.x * W + f].x, a, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&partials[(size_t)blockIdx.x * W + f].y, b, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __threadfence();
+  __syncthreads();
+  if (t == 0) s_last = (atomicAdd(&counter[0], 1u) == gridDim.x - 1);
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  double a = 0.0, b = 0.0;
+  for (uint32_t blk = g; blk < gridDim.x; blk += G) {
+    a += __hip_atomic_load(&partials[(size_t)blk * W + f].x, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    b += __hip_atomic_load(&partials[(size_t)blk * W + f].y, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  s_ll[t] = a;
+  s_tot[t] = b;
+  __syncthreads();
+  if (t < (uint32_t)W) {
+    double sa = 0.0, sb = 0.0;
+    for (int k = 0; k < G; ++k) {
+      sa += s_ll[k * W + f];
+      sb += s_tot[k * W + f];
+    }
+    const unsigned slot = counter[1] % ring;
+    results[(size_t)slot * W + f] = make_double2(sa, sb);
+  }
+  __syncthreads();
+  if (t == 0) {
+    atomicAdd(&counter[1], 1u);
+    atomicExch(&counter[0], 0u);
+  }
+}
+
+// Sequential fp32 dot product of one segment's staged records with W
+// interleaved right-hand sides (lane f: frame f), multiply and add rounded
+// separately, four gathers in flight.
+template <int W>
+__device__ __forceinline__ float seg_dot_batch(const uint2* rec, uint32_t b, uint32_t e,
+                                               const float* __restrict__ v, uint32_t f) {
+  float acc = 0.f;
+  uint32_t k = b;
+  for (; k + 4 <= e; k += 4) {
+    const uint2 r0 = rec[k], r1 = rec[k + 1], r2 = rec[k + 2], r3 = rec[k + 3];
+    const float g0 = v[(size_t)r0.x * W + f], g1 = v[(size_t)r1.x * W + f];
+    const float g2 = v[(size_t)r2.x * W + f], g3 = v[(size_t)r3.x * W + f];
+    acc = acc + __uint_as_float(r0.y) * g0;
+    acc = acc + __uint_as_float(r1.y) * g1;
+    acc = acc + __uint_as_float(r2.y) * g2;
+    acc = acc + __uint_as_float(r3.y) * g3;
+  }
+  for (; k < e; ++k) {
+    const uint2 r = rec[k];
+    acc = acc + __uint_as_float(r.y) * v[(size_t)r.x * W + f];
+  }
+  return acc;
+}
+
+// Forward projection + ratio (+ per-frame statistics) for one RHS block.
+// Persistent grid over the row tiles.
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_fwd_batch(
+    const uint32_t* __restrict__ rowptr, const uint2* __restrict__ csr,
+    const uint2* __restrict__ tiles, uint32_t ntiles, const float* __restrict__ X,
+    const int32_t* __restrict__ Y, float* __restrict__ R, double2* __restrict__ partials,
+    unsigned* __restrict__ counter, double2* __restrict__ results, unsigned ring) {
+  constexpr int G = kBlock / W;
+  __shared__ uint2 rec[kTile];
+  __shared__ uint32_t S[kBlock + 1];
+  const uint32_t t = threadIdx.x, f = t % W, grp = t / W;
+  double ll = 0.0, tot = 0.0;
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint2 d0 = tiles[tile], d1 = tiles[tile + 1];
+    const uint32_t s0 = d0.x, e0 = d0.y, nseg = d1.x - s0, count = d1.y - e0;
+    __syncthreads();
+    const bool staged = count <= kTile;
+    if (staged) {
+      for (uint32_t i = t; i < count; i += kBlock) rec[i] = csr[e0 + i];
+    }
+    for (uint32_t i = t; i <= nseg; i += kBlock) S[i] = rowptr[s0 + i] - e0;
+    __syncthreads();
+    for (uint32_t sg = grp; sg < nseg; sg += G) {
+      const uint32_t row = s0 + sg;
+      const float F = staged ? seg_dot_batch<W>(rec, S[sg], S[sg + 1], X, f)
+                             : seg_dot_batch<W>(csr + e0, S[sg], S[sg + 1], X, f);
+      const int32_t yi = Y[(size_t)row * W + f];
+      R[(size_t)row * W + f] = (F != 0.f) ? (float)yi / F : 0.f;
+      tot += (double)F;
+      if (F > 0.f) ll += (yi != 0 ? (double)yi * log((double)F) : 0.0) - (double)F;
+    }
+  }
+  batch_stats_finish<W>(ll, tot, partials, counter, results, ring);
+}
+
+// Backprojection + update for one RHS block over the CSC column tiles; lanes
+// whose frame is finished (bit clear in *mask) keep their image.
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_bwd_batch(
+    const uint32_t* __restrict__ colptr, const uint2* __restrict__ csc,
+    const uint2* __restrict__ tiles, uint32_t ntiles, const float* __restrict__ R,
+    const float* __restrict__ norm, float* __restrict__ X, const uint32_t* __restrict__ mask) {
+  constexpr int G = kBlock / W;
+  __shared__ uint2 rec[kTile];
+  __shared__ uint32_t S[kBlock + 1];
+  const uint32_t t = threadIdx.x, f = t % W, grp = t / W;
+  const bool live = (*mask >> f) & 1u;
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint2 d0 = tiles[tile], d1 = tiles[tile + 1];
+    const uint32_t s0 = d0.x, e0 = d0.y, nseg = d1.x - s0, count = d1.y - e0;
+    __syncthreads();
+    const bool staged = count <= kTile;
+    if (staged) {
+      for (uint32_t i = t; i < count; i += kBlock) rec[i] = csc[e0 + i];
+    }
+    for (uint32_t i = t; i <= nseg; i += kBlock) S[i] = colptr[s0 + i] - e0;
+    __syncthreads();
+    for (uint32_t sg = grp; sg < nseg; sg += G) {
+      const uint32_t col = s0 + sg;
+      const float u = staged ? seg_dot_batch<W>(rec, S[sg], S[sg + 1], R, f)
+                             : seg_dot_batch<W>(csc + e0, S[sg], S[sg + 1], R, f);
+      if (live) {
+        const float n = norm[col];
+        const size_t i = (size_t)col * W + f;
+        X[i] = X[i] * ((n != 0.f) ? u / n : u);
+      }
+    }
+  }
+}
+
+// out[c*W + lane] = in[src(c)] for one frame: file-order image <-> block lane.
+__global__ __launch_bounds__(kBlock) void k_lane_in(float* __restrict__ X, int W, int lane,
+                                                    const float* __restrict__ file_img,
+                                                    const uint32_t* __restrict__ colmap,
+                                                    uint32_t cols) {
+  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= cols) return;
+  const uint32_t d = colmap ? colmap[c] : c;
+  X[(size_t)d * W + lane] = file_img[c];
+}
+
+__global__ __launch_bounds__(kBlock) void k_lane_out(const float* __restrict__ X, int W, int lane,
+                                                     float* __restrict__ file_img,
+                                                     const uint32_t* __restrict__ colmap,
+                                                     uint32_t cols) {
+  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= cols) return;
+  const uint32_t d = colmap ? colmap[c] : c;
+  file_img[c] = X[(size_t)d * W + lane];
+}
+
+__global__ __launch_bounds__(kBlock) void k_lane_fill(float* __restrict__ X, int W, int lane,
+                                                      float v, uint32_t cols) {
+  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+  if (c < cols) X[(size_t)c * W + lane] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_lane_y(int32_t* __restrict__ Y, int W, int lane,
+                                                   const int32_t* __restrict__ file_y,
+                                                   const uint32_t* __restrict__ rowold,
+                                                   uint32_t rows) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= rows) return;
+  Y[(size_t)i * W + lane] = file_y[rowold ? rowold[i] : i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_lane_move(const float* __restrict__ src, int Ws,
+                                                      int ls, float* __restrict__ dst, int Wd,
+                                                      int ld, uint32_t n) {
+  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+  if (c < n) dst[(size_t)c * Wd + ld] = src[(size_t)c * Ws + ls];
+}
+
+__global__ __launch_bounds__(kBlock) void k_lane_move_i32(const int32_t* __restrict__ src, int Ws,
+                                                          int ls, int32_t* __restrict__ dst,
+                                                          int Wd, int ld, uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) dst[(size_t)i * Wd + ld] = src[(size_t)i * Ws + ls];
+}
+
+std::vector<uint32_t> partition(const std::vector<uint32_t>& ptr, uint32_t T) {
+  const uint32_t n = (uint32_t)ptr.size() - 1;
+  std::vector<uint32_t> tiles{0};
+  uint32_t s = 0;
+  while (s < n) {
+    uint32_t e = s;
+    if (ptr[s + 1] - ptr[s] > T) {
+      e = s + 1;
+    } else {
+      uint64_t cnt = 0;
+      while (e < n && e - s < (uint32_t)kBlock && cnt + (ptr[e + 1] - ptr[e]) <= T) {
+        cnt += ptr[e + 1] - ptr[e];
+        ++e;
+      }
+    }
+    tiles.push_back(e);
+    s = e;
+  }
+  return tiles;
+}
+
+template <class T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+struct RhsBlock {
+  int W = 0;
+  std::vector<int> frames;  // frame in each lane; -1 = idle lane
+  float* X = nullptr;       // [cols][W], device voxel order
+  int32_t* Y = nullptr;     // [rows][W], device row order
+  float* R = nullptr;       // [rows][W]
+  uint32_t* mask = nullptr; // bit f set: lane f still iterating
+  double2* partials = nullptr;
+  unsigned* counter = nullptr;
+  double2* results = nullptr;  // [kRing][W]
+  unsigned iters = 0;          // iterations run by this block (ring slots used)
+  ~RhsBlock() {
+    dfree(X);
+    dfree(Y);
+    dfree(R);
+    dfree(mask);
+    dfree(partials);
+    dfree(counter);
+    dfree(results);
+  }
+};
+
+}  // namespace
+
+struct mlem_batch {
+  mlem_matrix* mat = nullptr;
+  uint32_t F = 0;
+  int wmax = 32;
+  double tol = 0.0;
+  bool use_graph = false;
+  std::vector<int> budget, done;
+  std::vector<std::vector<double>> ll;   // per frame: log-likelihood of x_{k-1}, k = 1..done
+  std::vector<char> active;
+  std::vector<std::unique_ptr<RhsBlock>> blocks;
+  float* final_x = nullptr;    // [F][cols] file order: images of finished frames
+  int32_t* y_file = nullptr;   // [F][rows] file order (kept for repacking)
+  uint2* tiles_f = nullptr;
+  uint2* tiles_b = nullptr;
+  uint32_t nt_f = 0, nt_b = 0;
+  uint32_t grid = 0;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  uint64_t frame_iterations = 0;
+  ~mlem_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    blocks.clear();
+    dfree(final_x);
+    dfree(y_file);
+    dfree(tiles_f);
+    dfree(tiles_b);
+  }
+};
+
+namespace {
+
+void drop_graph(mlem_batch* B) {
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
+  B->exec = nullptr;
+  B->graph = nullptr;
+}
+
+std::unique_ptr<RhsBlock> new_block(mlem_batch* B, int W) {
+  const DevMatrix& m = B->mat->m;
+  std::unique_ptr<RhsBlock> b(new RhsBlock());
+  b->W = W;
+  b->frames.assign(W, -1);
+  MLEM_HIP(hipMalloc(&b->X, 4ull * m.cols * W));
+  MLEM_HIP(hipMalloc(&b->Y, 4ull * m.rows * W));
+  MLEM_HIP(hipMalloc(&b->R, 4ull * m.rows * W));
+  MLEM_HIP(hipMalloc(&b->mask, 4));
+  MLEM_HIP(hipMalloc(&b->partials, sizeof(double2) * (size_t)B->grid * W));
+  MLEM_HIP(hipMalloc(&b->counter, 2 * sizeof(unsigned)));
+  MLEM_HIP(hipMalloc(&b->results, sizeof(double2) * (size_t)kRing * W));
+  MLEM_HIP(hipMemsetAsync(b->counter, 0, 2 * sizeof(unsigned), m.stream));
+  MLEM_HIP(hipMemsetAsync(b->X, 0, 4ull * m.cols * W, m.stream));
+  MLEM_HIP(hipMemsetAsync(b->Y, 0, 4ull * m.rows * W, m.stream));
+  return b;
+}
+
+void upload_mask(mlem_batch* B, RhsBlock& b) {
+  uint32_t mask = 0;
+  for (int l = 0; l < b.W; ++l)
+    if (b.frames[l] >= 0 && B->active[b.frames[l]]) mask |= 1u << l;
+  MLEM_HIP(hipMemcpyAsync(b.mask, &mask, 4, hipMemcpyHostToDevice, B->mat->m.stream));
+  MLEM_HIP(hipStreamSynchronize(B->mat->m.stream));
+}
+
+template <int W>
+void launch_block(mlem_batch* B, RhsBlock& b, hipStream_t s) {
+  const DevMatrix& m = B->mat->m;
+  hipLaunchKernelGGL((k_fwd_batch<W>), dim3(B->grid), dim3(kBlock), 0, s, m.rowptr, m.csr,
+                     B->tiles_f, B->nt_f, b.X, b.Y, b.R, b.partials, b.counter, b.results, kRing);
+  hipLaunchKernelGGL((k_bwd_batch<W>), dim3(B->grid), dim3(kBlock), 0, s, m.colptr, m.csc,
+                     B->tiles_b, B->nt_b, b.R, m.norm, b.X, b.mask);
+}
+
+void enqueue_iteration(mlem_batch* B, hipStream_t s) {
+  for (auto& bp : B->blocks) {
+    switch (bp->W) {
+      case 4: launch_block<4>(B, *bp, s); break;
+      case 8: launch_block<8>(B, *bp, s); break;
+      case 16: launch_block<16>(B, *bp, s); break;
+      case 32: launch_block<32>(B, *bp, s); break;
+      default: raise(MLEM_EINVAL, "unsupported block width");
+    }
+  }
+  check_launch("batched iteration launch");
+}
+
+// Place the still-active frames into the fewest blocks of the narrowest
+// width that holds them; finished frames' images move to final_x.
+void repack(mlem_batch* B, bool force) {
+  DevMatrix& m = B->mat->m;
+  int A = 0;
+  for (uint32_t f = 0; f < B->F; ++f) A += B->active[f] ? 1 : 0;
+  int W = 4;
+  while (W < B->wmax && W < A) W *= 2;
+  const size_t nb = A == 0 ? 0 : (size_t)((A + W - 1) / W);
+  bool same = !force && nb == B->blocks.size();
+  for (auto& b : B->blocks) same = same && b->W == W;
+  // save finished frames still living in a block
+  for (auto& b : B->blocks)
+    for (int l = 0; l < b->W; ++l) {
+      const int f = b->frames[l];
+      if (f >= 0 && !B->active[f]) {
+        hipLaunchKernelGGL(k_lane_out, dim3(blocks_for(m.cols)), dim3(kBlock), 0, m.stream, b->X,
+                           b->W, l, B->final_x + (size_t)f * m.cols, m.colmap, m.cols);
+        if (same) b->frames[l] = -1;
+      }
+    }
+  check_launch("lane copy");
+  if (same) {
+    for (auto& b : B->blocks) upload_mask(B, *b);
+    return;
+  }
+  drop_graph(B);
+  std::vector<std::unique_ptr<RhsBlock>> nbk;
+  for (size_t i = 0; i < nb; ++i) nbk.push_back(new_block(B, W));
+  size_t slot = 0;
+  for (uint32_t f = 0; f < B->F; ++f) {
+    if (!B->active[f]) continue;
+    RhsBlock& d = *nbk[slot / W];
+    const int l = (int)(slot % W);
+    d.frames[l] = (int)f;
+    // find the frame's current lane (if any)
+    bool moved = false;
+    for (auto& b : B->blocks)
+      for (int ls = 0; ls < b->W && !moved; ++ls)
+        if (b->frames[ls] == (int)f) {
+          hipLaunchKernelGGL(k_lane_move, dim3(blocks_for(m.cols)), dim3(kBlock), 0, m.stream,
+                             b->X, b->W, ls, d.X, d.W, l, m.cols);
+          hipLaunchKernelGGL(k_lane_move_i32, dim3(blocks_for(m.rows)), dim3(kBlock), 0,
+                             m.stream, b->Y, b->W, ls, d.Y, d.W, l, m.rows);
+          moved = true;
+        }
+    if (!moved) raise(MLEM_ESTATE, "batched frame lost during repacking");
+    ++slot;
+  }
+  check_launch("repack");
+  MLEM_HIP(hipStreamSynchronize(m.stream));
+  B->blocks = std::move(nbk);
+  for (auto& b : B->blocks) upload_mask(B, *b);
+}
+
+}  // namespace
+
+extern "C" {
+
+void mlem_batch_options_default(mlem_batch_options* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->rel_tol = 0.0;
+  o->block_width = 32;
+  o->use_graph = 1;
+}
+
+int mlem_batch_create(mlem_matrix* h, uint32_t frames, const int32_t* y, uint32_t rows,
+                      const int32_t* budgets, const mlem_batch_options* opts, mlem_batch** out) {
+  return guarded([&] {
+    if (!h || !out || !y || frames == 0) raise(MLEM_EINVAL, "bad argument");
+    *out = nullptr;
+    DevMatrix& m = h->m;
+    if (rows != m.rows) raise(MLEM_EINVAL, "sinogram length does not match the matrix rows");
+    if (!m.csc) raise(MLEM_ESTATE, "batched reconstruction needs the CSC transpose");
+    if (m.vk != kF32) raise(MLEM_EINVAL, "batched reconstruction needs fp32 matrix values");
+    mlem_batch_options o;
+    if (opts)
+      o = *opts;
+    else
+      mlem_batch_options_default(&o);
+    if (o.block_width != 4 && o.block_width != 8 && o.block_width != 16 && o.block_width != 32)
+      raise(MLEM_EINVAL, "block_width must be 4, 8, 16 or 32");
+    std::unique_ptr<mlem_batch> B(new mlem_batch());
+    B->mat = h;
+    B->F = frames;
+    B->wmax = o.block_width;
+    B->tol = o.rel_tol;
+    B->use_graph = o.use_graph != 0;
+    B->budget.assign(frames, 0x7FFFFFFF);
+    if (budgets)
+      for (uint32_t f = 0; f < frames; ++f) {
+        if (budgets[f] < 0) raise(MLEM_EINVAL, "negative iteration budget");
+        B->budget[f] = budgets[f];
+      }
+    B->done.assign(frames, 0);
+    B->ll.assign(frames, {});
+    B->active.assign(frames, 1);
+    for (uint32_t f = 0; f < frames; ++f)
+      if (B->budget[f] == 0) B->active[f] = 0;
+    B->grid = (uint32_t)std::max(8, 8 * m.cus);
+    // tiles (device-order pointers are on the host after open)
+    prepare_tiles(m, m.stream);
+    const std::vector<uint32_t> tf = partition(m.h_rowptr, kTile), tb = partition(m.h_colptr, kTile);
+    std::vector<uint2> df(tf.size()), db(tb.size());
+    for (size_t i = 0; i < tf.size(); ++i) df[i] = make_uint2(tf[i], m.h_rowptr[tf[i]]);
+    for (size_t i = 0; i < tb.size(); ++i) db[i] = make_uint2(tb[i], m.h_colptr[tb[i]]);
+    B->nt_f = (uint32_t)tf.size() - 1;
+    B->nt_b = (uint32_t)tb.size() - 1;
+    MLEM_HIP(hipMalloc(&B->tiles_f, sizeof(uint2) * df.size()));
+    MLEM_HIP(hipMalloc(&B->tiles_b, sizeof(uint2) * db.size()));
+    MLEM_HIP(hipMemcpyAsync(B->tiles_f, df.data(), sizeof(uint2) * df.size(),
+                            hipMemcpyHostToDevice, m.stream));
+    MLEM_HIP(hipMemcpyAsync(B->tiles_b, db.data(), sizeof(uint2) * db.size(),
+                            hipMemcpyHostToDevice, m.stream));
+    MLEM_HIP(hipMalloc(&B->final_x, 4ull * m.cols * frames));
+    MLEM_HIP(hipMalloc(&B->y_file, 4ull * m.rows * frames));
+    MLEM_HIP(hipMemcpyAsync(B->y_file, y, 4ull * m.rows * frames, hipMemcpyHostToDevice,
+                            m.stream));
+    // x0 per frame: (float)(sum y / sum norm), initImage (seqcsr4mlem.cpp:52-66)
+    std::vector<float> x0(frames);
+    for (uint32_t f = 0; f < frames; ++f) {
+      double s = 0.0;
+      const int32_t* yf = y + (size_t)f * rows;
+      for (uint32_t i = 0; i < rows; ++i) s += yf[i];
+      x0[f] = (float)(s / m.sumnorm);
+      hipLaunchKernelGGL(k_lane_fill, dim3(blocks_for(m.cols)), dim3(kBlock), 0, m.stream,
+                         B->final_x + (size_t)f * m.cols, 1, 0, x0[f], m.cols);
+    }
+    // initial blocks: every frame with a budget
+    int A = 0;
+    for (uint32_t f = 0; f < frames; ++f) A += B->active[f];
+    int W = 4;
+    while (W < B->wmax && W < A) W *= 2;
+    const size_t nb = A == 0 ? 0 : (size_t)((A + W - 1) / W);
+    size_t slot = 0;
+    for (size_t i = 0; i < nb; ++i) B->blocks.push_back(new_block(B.get(), W));
+    for (uint32_t f = 0; f < frames; ++f) {
+      if (!B->active[f]) continue;
+      RhsBlock& b = *B->blocks[slot / W];
+      const int l = (int)(slot % W);
+      b.frames[l] = (int)f;
+      hipLaunchKernelGGL(k_lane_fill, dim3(blocks_for(m.cols)), dim3(kBlock), 0, m.stream, b.X,
+                         b.W, l, x0[f], m.cols);
+      hipLaunchKernelGGL(k_lane_y, dim3(blocks_for(m.rows)), dim3(kBlock), 0, m.stream, b.Y, b.W,
+                         l, B->y_file + (size_t)f * m.rows, m.rowold, m.rows);
+      ++slot;
+    }
+    check_launch("batch setup");
+    for (auto& b : B->blocks) upload_mask(B.get(), *b);
+    MLEM_HIP(hipStreamSynchronize(m.stream));
+    *out = B.release();
+  });
+}
+
+int mlem_batch_run(mlem_batch* B, int max_iterations, int* iterations_run) {
+  return guarded([&] {
+    if (!B || max_iterations < 0) raise(MLEM_EINVAL, "bad argument");
+    DevMatrix& m = B->mat->m;
+    int it = 0;
+    for (; it < max_iterations && !B->blocks.empty(); ++it) {
+      if (B->use_graph) {
+        if (!B->exec) {
+          MLEM_HIP(hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal));
+          try {
+            enqueue_iteration(B, m.stream);
+          } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(m.stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+          }
+          MLEM_HIP(hipStreamEndCapture(m.stream, &B->graph));
+          MLEM_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+        }
+        MLEM_HIP(hipGraphLaunch(B->exec, m.stream));
+      } else {
+        enqueue_iteration(B, m.stream);
+      }
+      // per-frame statistics of this iteration
+      bool changed = false;
+      for (auto& b : B->blocks) {
+        std::vector<double2> r(b->W);
+        const unsigned slot = b->iters % kRing;
+        MLEM_HIP(hipMemcpyAsync(r.data(), b->results + (size_t)slot * b->W,
+                                sizeof(double2) * b->W, hipMemcpyDeviceToHost, m.stream));
+        MLEM_HIP(hipStreamSynchronize(m.stream));
+        b->iters++;
+        for (int l = 0; l < b->W; ++l) {
+          const int f = b->frames[l];
+          if (f < 0 || !B->active[f]) continue;
+          B->done[f]++;
+          B->frame_iterations++;
+          B->ll[f].push_back(r[l].x);
+          const int k = B->done[f];
+          bool stop = k >= B->budget[f];
+          if (!stop && B->tol > 0.0 && k >= 2) {
+            const double a = B->ll[f][k - 1], p = B->ll[f][k - 2];
+            stop = std::fabs(a - p) <= B->tol * std::fabs(a);
+          }
+          if (stop) {
+            B->active[f] = 0;
+            changed = true;
+          }
+        }
+      }
+      if (changed) repack(B, false);
+    }
+    if (iterations_run) *iterations_run = it;
+  });
+}
+
+int mlem_batch_image(mlem_batch* B, uint32_t frame, float* x, uint32_t cols) {
+  return guarded([&] {
+    if (!B || !x || frame >= B->F) raise(MLEM_EINVAL, "bad argument");
+    DevMatrix& m = B->mat->m;
+    if (cols != m.cols) raise(MLEM_EINVAL, "image size does not match the matrix columns");
+    float* src = B->final_x + (size_t)frame * m.cols;
+    for (auto& b : B->blocks)
+      for (int l = 0; l < b->W; ++l)
+        if (b->frames[l] == (int)frame)
+          hipLaunchKernelGGL(k_lane_out, dim3(blocks_for(m.cols)), dim3(kBlock), 0, m.stream,
+                             b->X, b->W, l, src, m.colmap, m.cols);
+    check_launch("image copy");
+    MLEM_HIP(hipMemcpyAsync(x, src, 4ull * cols, hipMemcpyDeviceToHost, m.stream));
+    MLEM_HIP(hipStreamSynchronize(m.stream));
+  });
+}
+
+int mlem_batch_frame_info(mlem_batch* B, uint32_t frame, int* iterations, double* loglik,
+                          int capacity) {
+  return guarded([&] {
+    if (!B || frame >= B->F) raise(MLEM_EINVAL, "bad argument");
+    if (iterations) *iterations = B->done[frame];
+    if (loglik)
+      for (int k = 0; k < capacity && k < (int)B->ll[frame].size(); ++k) loglik[k] = B->ll[frame][k];
+  });
+}
+
+int mlem_batch_progress(const mlem_batch* B, uint32_t* active_frames, uint64_t* frame_iterations,
+                        int* block_width, uint32_t* blocks) {
+  return guarded([&] {
+    if (!B) raise(MLEM_EINVAL, "null handle");
+    uint32_t a = 0;
+    for (char c : B->active) a += c ? 1 : 0;
+    if (active_frames) *active_frames = a;
+    if (frame_iterations) *frame_iterations = B->frame_iterations;
+    if (block_width) *block_width = B->blocks.empty() ? 0 : B->blocks[0]->W;
+    if (blocks) *blocks = (uint32_t)B->blocks.size();
+  });
+}
+
+void mlem_batch_destroy(mlem_batch* B) {
+  if (!B) return;
+  (void)hipStreamSynchronize(B->mat->m.stream);
+  delete B;
+}
+
+}  // extern "C"

@@ -4,8 +4,12 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
+
+#include "csr4.hpp"
 
 namespace mlem {
 
@@ -19,6 +23,35 @@ struct ApiError {
 void hip_check(hipError_t e, const char* what);
 #define MLEM_HIP(call) ::mlem::hip_check((call), #call)
 
+// Thread-local message of the last failing C-ABI call (api.cpp).
+void set_error(const std::string& msg);
+void clear_error();
+
+// Run f and translate exceptions into a C-ABI status code.
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    clear_error();
+    return 0;
+  } catch (const ApiError& e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const Csr4Error& e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed");
+    return -4;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return -1;
+  } catch (...) {
+    set_error("unknown error");
+    return -1;
+  }
+}
+
 // FAST-mode kernel family of a projection.
 enum KernelKind { kGroups = 0, kTiles = 1, kDict = 2 };
 
@@ -26,14 +59,18 @@ enum KernelKind { kGroups = 0, kTiles = 1, kDict = 2 };
 // time for the elements-per-lane setting `ept`; 0 = not built).
 struct DictFormat {
   int ept = 0;
+  int vk = 0;                 // value storage (ValueKind): fp32, fp16, bf16
   uint32_t ntiles = 0;
   uint4* tiles = nullptr;     // [ntiles+1] {first segment, first element,
                               //  aligned element start, first dict entry}
   uint32_t* dict = nullptr;   // distinct gather indices, sorted per tile
-  float* dval = nullptr;      // element values, tile-aligned
+  void* dval = nullptr;       // element values, tile-aligned (4 or 2 bytes)
   uint16_t* dlix = nullptr;   // element positions in the tile dictionary
   uint64_t ndict = 0, nval = 0;
 };
+
+// Storage precision of the matrix values.
+enum ValueKind { kF32 = 0, kF16 = 1, kBF16 = 2 };
 
 // Launch configuration of one projection (forward or backward).
 struct GroupLaunch {
@@ -64,6 +101,12 @@ struct DevMatrix {
   uint32_t* rowold = nullptr;  // [rows]
   uint32_t* rowpos = nullptr;  // [rows]
   double sumnorm = 0.0;        // sequential double sum of norm (initImage)
+
+  // Reduced-precision values: every record holds q_ij (exactly representable
+  // in fp16 / bf16) and a_ij = rscale[i] * q_ij. The projections run on q;
+  // the scales enter the FP64 column sums and the FP64 statistics only.
+  int vk = kF32;
+  float* rscale = nullptr;     // [rows] device row order, nullptr for fp32
 
   GroupLaunch fwd, bwd, scatter;
 
@@ -128,6 +171,13 @@ void launch_correl(uint32_t rows, const float* f, const int32_t* y, float* r,
 void launch_colsums_csc(const DevMatrix& m, float* norm, hipStream_t s);
 // norm by atomic scatter (matrices opened without a CSC).
 void launch_colsums_atomic(const DevMatrix& m, float* norm, hipStream_t s);
+// Reduced precision: row scales (file row order) and in-place quantization
+// of the file-order CSR values to fp16 / bf16 (kept as exact fp32 words).
+void launch_quantize(DevMatrix& m, int vk, float* rscale, hipStream_t s);
+// norm_j = (float) sum_i (double) s_i q_ij from the CSC, ascending row order.
+void launch_colsums_scaled(const DevMatrix& m, float* norm, hipStream_t s);
+// r_i *= s_i in device row order (no-op for fp32 values).
+void launch_scale_rows(const DevMatrix& m, float* r, hipStream_t s);
 // ---- load-time layout (layout.hip) ----------------------------------------
 // Renumber rows (sinogram order) and voxels (4x4x2 bricks) when the grid is
 // known, build the CSC (file row order inside every column) and the
@@ -152,3 +202,12 @@ void launch_fill(float* p, float v, size_t n, hipStream_t s);
 void launch_copy(const uint4* src, uint4* dst, size_t n16, hipStream_t s);
 
 }  // namespace mlem
+
+// The C-ABI matrix handle (include/mlem.h): the device matrix plus load
+// timings. Defined here so every source of the library can reach it.
+struct mlem_matrix {
+  mlem::DevMatrix m;
+  double norm_seconds = 0.0;  // layout, CSC build and column sums at open
+  double load_seconds = 0.0;  // file/host -> HBM
+  ~mlem_matrix();
+};

@@ -146,19 +146,71 @@ __device__ void stats_finish(double ll, double tot, double2* __restrict__ partia
   }
 }
 
+// Forward epilogue: r = (f != 0) ? y/f : 0 (or f itself), optional FP64
+// log-likelihood and sum of f. With reduced-precision values the row sum is
+// q_i.x and the projection is s_i * (q_i.x); the ratio y_i / (q_i.x) already
+// carries the factor s_i that the backprojection of q needs, so s_i only
+// enters the statistics and the plain projection.
+template <bool RATIO, bool STATS>
+struct FwdEpilogue {
+  const int32_t* __restrict__ y;
+  float* __restrict__ out;
+  const float* __restrict__ rscale = nullptr;
+  double ll = 0.0, tot = 0.0;
+  int32_t yi = 0;
+  float s = 1.f;
+  __device__ void prefetch(uint32_t row) {
+    if constexpr (RATIO) yi = y[row];
+    if (rscale && (STATS || !RATIO)) s = rscale[row];
+  }
+  __device__ void finish(uint32_t row, float f) {
+    if constexpr (RATIO) {
+      out[row] = (f != 0.f) ? (float)yi / f : 0.f;
+      if constexpr (STATS) {
+        const double fd = (double)f * (double)s;
+        tot += fd;
+        if (fd > 0.0) ll += (yi != 0 ? (double)yi * log(fd) : 0.0) - fd;
+      }
+    } else {
+      out[row] = rscale ? f * s : f;
+    }
+  }
+};
+
+// Backward epilogue: x *= (n != 0) ? u/n : u (or u itself).
+template <bool UPDATE>
+struct BwdEpilogue {
+  const float* __restrict__ norm;
+  float* __restrict__ out;
+  float n = 0.f, x = 0.f;
+  __device__ void prefetch(uint32_t col) {
+    if constexpr (UPDATE) {
+      n = norm[col];
+      x = out[col];
+    }
+  }
+  __device__ void finish(uint32_t col, float u) {
+    if constexpr (UPDATE)
+      out[col] = x * ((n != 0.f) ? u / n : u);
+    else
+      out[col] = u;
+  }
+};
+
 // ---------------------------------------------------------------- forward --
 template <int G, bool EXACT, bool RATIO, bool STATS>
 __global__ __launch_bounds__(kBlock) void k_fwd(
     const uint32_t* __restrict__ rowptr, const uint2* __restrict__ csr,
     const float* __restrict__ x, const int32_t* __restrict__ y, float* __restrict__ out,
-    uint32_t rows, uint32_t chunk, double2* __restrict__ partials, unsigned* __restrict__ counter,
+    const float* __restrict__ rscale, uint32_t rows, uint32_t chunk,
+    double2* __restrict__ partials, unsigned* __restrict__ counter,
     double2* __restrict__ results, unsigned ring) {
   constexpr uint32_t GPB = kBlock / G;
   const uint32_t lane = threadIdx.x % G;
   const uint32_t grp = threadIdx.x / G;
   const uint64_t r0 = (uint64_t)logical_block() * chunk;
   const uint64_t r1 = r0 + chunk < rows ? r0 + chunk : rows;
-  double ll = 0.0, tot = 0.0;
+  FwdEpilogue<RATIO, STATS> epi{y, out, rscale};
   for (uint64_t base = r0; base < r1; base += GPB) {
     const uint32_t row = (uint32_t)(base + grp);
     const bool active = base + grp < r1;
@@ -166,19 +218,11 @@ __global__ __launch_bounds__(kBlock) void k_fwd(
     if (active) acc = seg_dot<G, EXACT>(csr, x, rowptr[row], rowptr[row + 1], lane);
     acc = group_sum<G>(acc);
     if (active && lane == 0) {
-      if constexpr (RATIO) {
-        const int32_t yi = y[row];
-        out[row] = (acc != 0.f) ? (float)yi / acc : 0.f;
-        if constexpr (STATS) {
-          tot += (double)acc;
-          if (acc > 0.f) ll += (yi != 0 ? (double)yi * log((double)acc) : 0.0) - (double)acc;
-        }
-      } else {
-        out[row] = acc;
-      }
+      epi.prefetch(row);
+      epi.finish(row, acc);
     }
   }
-  if constexpr (STATS) stats_finish(ll, tot, partials, counter, results, ring);
+  if constexpr (STATS) stats_finish(epi.ll, epi.tot, partials, counter, results, ring);
 }
 
 // --------------------------------------------------------------- backward --
@@ -299,58 +343,15 @@ __device__ __forceinline__ void tile_spmv(const uint32_t* __restrict__ segptr,
   if (t < nseg) epi.finish(s0 + t, Sum[t]);
 }
 
-// Forward epilogue: r = (f != 0) ? y/f : 0 (or f itself), optional FP64
-// log-likelihood and sum of f.
-template <bool RATIO, bool STATS>
-struct FwdEpilogue {
-  const int32_t* __restrict__ y;
-  float* __restrict__ out;
-  double ll = 0.0, tot = 0.0;
-  int32_t yi = 0;
-  __device__ void prefetch(uint32_t row) {
-    if constexpr (RATIO) yi = y[row];
-  }
-  __device__ void finish(uint32_t row, float f) {
-    if constexpr (RATIO) {
-      out[row] = (f != 0.f) ? (float)yi / f : 0.f;
-      if constexpr (STATS) {
-        tot += (double)f;
-        if (f > 0.f) ll += (yi != 0 ? (double)yi * log((double)f) : 0.0) - (double)f;
-      }
-    } else {
-      out[row] = f;
-    }
-  }
-};
-
-// Backward epilogue: x *= (n != 0) ? u/n : u (or u itself).
-template <bool UPDATE>
-struct BwdEpilogue {
-  const float* __restrict__ norm;
-  float* __restrict__ out;
-  float n = 0.f, x = 0.f;
-  __device__ void prefetch(uint32_t col) {
-    if constexpr (UPDATE) {
-      n = norm[col];
-      x = out[col];
-    }
-  }
-  __device__ void finish(uint32_t col, float u) {
-    if constexpr (UPDATE)
-      out[col] = x * ((n != 0.f) ? u / n : u);
-    else
-      out[col] = u;
-  }
-};
-
 template <int EPT, bool RATIO, bool STATS>
 __global__ __launch_bounds__(kBlock) void k_fwd_tiles(
     const uint32_t* __restrict__ rowptr, const uint2* __restrict__ csr,
     const uint2* __restrict__ tiles, uint32_t ntiles, const float* __restrict__ x,
-    const int32_t* __restrict__ y, float* __restrict__ out, double2* __restrict__ partials,
-    unsigned* __restrict__ counter, double2* __restrict__ results, unsigned ring) {
+    const int32_t* __restrict__ y, float* __restrict__ out, const float* __restrict__ rscale,
+    double2* __restrict__ partials, unsigned* __restrict__ counter,
+    double2* __restrict__ results, unsigned ring) {
   const uint32_t tile = logical_block();
-  FwdEpilogue<RATIO, STATS> epi{y, out};
+  FwdEpilogue<RATIO, STATS> epi{y, out, rscale};
   if (tile < ntiles) tile_spmv<EPT>(rowptr, csr, tiles, tile, x, epi);
   if constexpr (STATS) stats_finish(epi.ll, epi.tot, partials, counter, results, ring);
 }
@@ -411,15 +412,41 @@ __device__ __forceinline__ float long_segment_sum(const uint2* __restrict__ el,
   return sum;
 }
 
-template <int EPT, class Epi>
+// Element values of a dictionary tile, loaded in pairs: two fp32 words, or
+// two 16-bit values (fp16 / bf16) in one word. The 16-bit forms widen
+// exactly to fp32.
+template <int VK>
+struct ValPair {
+  using W = u32x2;
+  __device__ static float lo(W w) { return __uint_as_float(w.x); }
+  __device__ static float hi(W w) { return __uint_as_float(w.y); }
+};
+template <>
+struct ValPair<kF16> {
+  using W = uint32_t;
+  __device__ static float lo(W w) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xFFFFu));
+  }
+  __device__ static float hi(W w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)); }
+};
+template <>
+struct ValPair<kBF16> {
+  using W = uint32_t;
+  __device__ static float lo(W w) { return __uint_as_float(w << 16); }
+  __device__ static float hi(W w) { return __uint_as_float(w & 0xFFFF0000u); }
+};
+
+template <int EPT, int VK, class Epi>
 __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
                                           const uint2* __restrict__ el,
                                           const uint4* __restrict__ tiles, uint32_t tile,
-                                          const float* __restrict__ dval,
+                                          const void* __restrict__ dval,
                                           const uint16_t* __restrict__ dlix,
                                           const uint32_t* __restrict__ dict,
                                           const float* __restrict__ v, Epi& epi) {
   static_assert(EPT % 2 == 0, "elements are loaded in pairs");
+  using VP = ValPair<VK>;
+  using W = typename VP::W;
   constexpr uint32_t T = kBlock * EPT;
   constexpr int H = EPT / 2;
   constexpr int G = 16;
@@ -440,10 +467,11 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
   }
   uint32_t send = 0;
   if (t < nseg) send = segptr[s0 + t + 1] - e0;
-  u32x2 val[H];
+  W val[H];
   uint32_t li[H];
   if (count > 0) {  // block-uniform
-    const u32x2* dv2 = reinterpret_cast<const u32x2*>(dval + a0);
+    const W* dv2 = reinterpret_cast<const W*>(
+        static_cast<const char*>(dval) + (size_t)a0 * (VK == kF32 ? 4 : 2));
     const uint32_t* dl2 = reinterpret_cast<const uint32_t*>(dlix + a0);
 #pragma unroll
     for (int j = 0; j < H; ++j) {
@@ -463,8 +491,8 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
 #pragma unroll
     for (int j = 0; j < H; ++j) {
       const uint32_t q = 2 * (t + j * kBlock);
-      P[q] = __uint_as_float(val[j].x) * V[li[j] & 0xFFFFu];
-      P[q + 1] = __uint_as_float(val[j].y) * V[li[j] >> 16];
+      P[q] = VP::lo(val[j]) * V[li[j] & 0xFFFFu];
+      P[q + 1] = VP::hi(val[j]) * V[li[j] >> 16];
     }
   }
   __syncthreads();
@@ -480,30 +508,31 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
   if (t < nseg) epi.finish(s0 + t, Sum[t]);
 }
 
-template <int EPT, bool RATIO, bool STATS>
+template <int EPT, int VK, bool RATIO, bool STATS>
 __global__ __launch_bounds__(kBlock) void k_fwd_dict(
     const uint32_t* __restrict__ rowptr, const uint2* __restrict__ csr,
-    const uint4* __restrict__ tiles, uint32_t ntiles, const float* __restrict__ dval,
+    const uint4* __restrict__ tiles, uint32_t ntiles, const void* __restrict__ dval,
     const uint16_t* __restrict__ dlix, const uint32_t* __restrict__ dict,
     const float* __restrict__ x, const int32_t* __restrict__ y, float* __restrict__ out,
-    double2* __restrict__ partials, unsigned* __restrict__ counter,
-    double2* __restrict__ results, unsigned ring) {
+    const float* __restrict__ rscale, double2* __restrict__ partials,
+    unsigned* __restrict__ counter, double2* __restrict__ results, unsigned ring) {
   const uint32_t tile = logical_block();
-  FwdEpilogue<RATIO, STATS> epi{y, out};
-  if (tile < ntiles) dict_spmv<EPT>(rowptr, csr, tiles, tile, dval, dlix, dict, x, epi);
+  FwdEpilogue<RATIO, STATS> epi{y, out, rscale};
+  if (tile < ntiles)
+    dict_spmv<EPT, VK>(rowptr, csr, tiles, tile, dval, dlix, dict, x, epi);
   if constexpr (STATS) stats_finish(epi.ll, epi.tot, partials, counter, results, ring);
 }
 
-template <int EPT, bool UPDATE>
+template <int EPT, int VK, bool UPDATE>
 __global__ __launch_bounds__(kBlock) void k_bwd_dict(
     const uint32_t* __restrict__ colptr, const uint2* __restrict__ csc,
-    const uint4* __restrict__ tiles, uint32_t ntiles, const float* __restrict__ dval,
+    const uint4* __restrict__ tiles, uint32_t ntiles, const void* __restrict__ dval,
     const uint16_t* __restrict__ dlix, const uint32_t* __restrict__ dict,
     const float* __restrict__ r, const float* __restrict__ norm, float* __restrict__ out) {
   const uint32_t tile = logical_block();
   if (tile >= ntiles) return;
   BwdEpilogue<UPDATE> epi{norm, out};
-  dict_spmv<EPT>(colptr, csc, tiles, tile, dval, dlix, dict, r, epi);
+  dict_spmv<EPT, VK>(colptr, csc, tiles, tile, dval, dlix, dict, r, epi);
 }
 
 // Dictionary construction, one block per tile. Pass 1 (tiles4 == nullptr)
@@ -511,13 +540,13 @@ __global__ __launch_bounds__(kBlock) void k_bwd_dict(
 // and the {value, position} element arrays. Keys (index << 16 | element
 // position) are bitonic-sorted in LDS; a block scan of "new index" flags
 // gives every element its dictionary position.
-template <int EPT>
+template <int EPT, int VK>
 __global__ __launch_bounds__(kBlock) void k_dict_build(const uint2* __restrict__ el,
                                                        const uint2* __restrict__ tiles2,
                                                        const uint4* __restrict__ tiles4,
                                                        uint32_t* __restrict__ nd_out,
                                                        uint32_t* __restrict__ dict,
-                                                       float* __restrict__ dval,
+                                                       void* __restrict__ dval,
                                                        uint16_t* __restrict__ dlix) {
   constexpr uint32_t T = kBlock * EPT;
   __shared__ unsigned long long K[T];
@@ -584,7 +613,14 @@ __global__ __launch_bounds__(kBlock) void k_dict_build(const uint2* __restrict__
     const uint32_t idx = (uint32_t)(K[q] >> 16), pos = (uint32_t)(K[q] & 0xFFFFu);
     if (q == 0 || (K[q] >> 16) != (K[q - 1] >> 16)) dict[g0 + rank++] = idx;
     dlix[a0 + pos] = (uint16_t)(rank - 1);
-    dval[a0 + pos] = __uint_as_float(el[e0 + pos].y);
+    const uint32_t bits = el[e0 + pos].y;  // fp32 bits; 16-bit kinds hold exact values
+    if constexpr (VK == kF32)
+      static_cast<uint32_t*>(dval)[a0 + pos] = bits;
+    else if constexpr (VK == kF16)
+      static_cast<uint16_t*>(dval)[a0 + pos] =
+          __builtin_bit_cast(uint16_t, (_Float16)__uint_as_float(bits));
+    else
+      static_cast<uint16_t*>(dval)[a0 + pos] = (uint16_t)(bits >> 16);
   }
 }
 
@@ -653,6 +689,67 @@ __global__ __launch_bounds__(kBlock) void k_colsum_atomic(const uint2* __restric
   }
 }
 
+// ------------------------------------------------- reduced-precision values --
+// Per-row scale s_i = max_k |a_ik| (1 for an all-zero row) and in-place
+// quantization of the row's values to q_ik = round(a_ik / s_i) in fp16
+// (round to nearest even, subnormals kept) or bf16 (round to nearest even on
+// the fp32 bits). The fp32 record keeps q_ik, which the 16-bit type holds
+// exactly. One 16-lane group per row, file row order.
+template <int VK>
+__device__ __forceinline__ float quantize_value(float v) {
+  if constexpr (VK == kF16) {
+    return (float)(_Float16)v;
+  } else {
+    const uint32_t u = __float_as_uint(v);
+    return __uint_as_float((u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u);
+  }
+}
+
+template <int VK>
+__global__ __launch_bounds__(kBlock) void k_quantize_rows(const uint32_t* __restrict__ rowptr,
+                                                          uint2* __restrict__ csr, uint32_t rows,
+                                                          float* __restrict__ rscale) {
+  constexpr int G = 16;
+  const uint64_t t = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
+  const uint64_t row = t / G;
+  const uint32_t lane = (uint32_t)(t % G);
+  const bool live = row < rows;  // the whole 16-lane group agrees
+  const uint32_t b = live ? rowptr[row] : 0u, e = live ? rowptr[row + 1] : 0u;
+  float mx = 0.f;
+  for (uint32_t k = b + lane; k < e; k += G) mx = fmaxf(mx, fabsf(__uint_as_float(csr[k].y)));
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, G));
+  if (!live) return;
+  const float s = (mx > 0.f) ? mx : 1.f;
+  for (uint32_t k = b + lane; k < e; k += G)
+    csr[k].y = __float_as_uint(quantize_value<VK>(__uint_as_float(csr[k].y) / s));
+  if (lane == 0) rscale[row] = s;
+}
+
+// norm_j = (float) sum_i (double) s_i * (double) q_ij, ascending file row
+// order (the CSC order); row indices are device rows, as is rscale.
+__global__ __launch_bounds__(kBlock) void k_colsum_scaled(const uint32_t* __restrict__ colptr,
+                                                          const uint2* __restrict__ csc,
+                                                          const float* __restrict__ rscale,
+                                                          float* __restrict__ norm, uint32_t cols) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= cols) return;
+  double s = 0.0;
+  for (uint32_t k = colptr[j]; k < colptr[j + 1]; ++k) {
+    const uint2 e = csc[k];
+    s += (double)rscale[e.x] * (double)__uint_as_float(e.y);
+  }
+  norm[j] = (float)s;
+}
+
+// r_i *= s_i (a backprojection of q needs s_i r_i).
+__global__ __launch_bounds__(kBlock) void k_scale_rows(float* __restrict__ r,
+                                                       const float* __restrict__ rscale,
+                                                       uint32_t rows) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < rows) r[i] = r[i] * rscale[i];
+}
+
 __global__ __launch_bounds__(kBlock) void k_fill(float* p, float v, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * kBlock)
@@ -705,13 +802,14 @@ void fwd_g(const DevMatrix& m, const float* x, const int32_t* y, float* out, boo
   const dim3 grid(g.grid), block(kBlock);
   if (stats)
     hipLaunchKernelGGL((k_fwd<G, EXACT, true, true>), grid, block, 0, s, m.rowptr, m.csr, x, y, out,
-                       m.rows, g.chunk, st->partials, st->counter, st->results, st->ring);
+                       m.rscale, m.rows, g.chunk, st->partials, st->counter, st->results,
+                       st->ring);
   else if (ratio)
     hipLaunchKernelGGL((k_fwd<G, EXACT, true, false>), grid, block, 0, s, m.rowptr, m.csr, x, y,
-                       out, m.rows, g.chunk, nullptr, nullptr, nullptr, 0u);
+                       out, m.rscale, m.rows, g.chunk, nullptr, nullptr, nullptr, 0u);
   else
     hipLaunchKernelGGL((k_fwd<G, EXACT, false, false>), grid, block, 0, s, m.rowptr, m.csr, x, y,
-                       out, m.rows, g.chunk, nullptr, nullptr, nullptr, 0u);
+                       out, m.rscale, m.rows, g.chunk, nullptr, nullptr, nullptr, 0u);
 }
 
 template <int G, bool EXACT>
@@ -745,14 +843,16 @@ void fwd_t(const DevMatrix& m, const float* x, const int32_t* y, float* out, boo
   if (stats && grid.x > st->max_blocks) raise(MLEM_ESTATE, "statistics buffer too small for launch");
   if (stats)
     hipLaunchKernelGGL((k_fwd_tiles<EPT, true, true>), grid, block, 0, s, m.rowptr, m.csr,
-                       m.tiles_fwd, m.ntiles_fwd, x, y, out, st->partials, st->counter,
+                       m.tiles_fwd, m.ntiles_fwd, x, y, out, m.rscale, st->partials, st->counter,
                        st->results, st->ring);
   else if (ratio)
     hipLaunchKernelGGL((k_fwd_tiles<EPT, true, false>), grid, block, 0, s, m.rowptr, m.csr,
-                       m.tiles_fwd, m.ntiles_fwd, x, y, out, nullptr, nullptr, nullptr, 0u);
+                       m.tiles_fwd, m.ntiles_fwd, x, y, out, m.rscale, nullptr, nullptr, nullptr,
+                       0u);
   else
     hipLaunchKernelGGL((k_fwd_tiles<EPT, false, false>), grid, block, 0, s, m.rowptr, m.csr,
-                       m.tiles_fwd, m.ntiles_fwd, x, y, out, nullptr, nullptr, nullptr, 0u);
+                       m.tiles_fwd, m.ntiles_fwd, x, y, out, m.rscale, nullptr, nullptr, nullptr,
+                       0u);
 }
 
 template <int EPT>
@@ -773,25 +873,50 @@ void check_launch(const char* what) {
   if (e != hipSuccess) raise(MLEM_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+template <int EPT, int VK>
+void fwd_dv(const DevMatrix& m, const float* x, const int32_t* y, float* out, bool ratio,
+            const StatsState* st, hipStream_t s) {
+  const DictFormat& D = m.dfwd;
+  const bool stats = ratio && st != nullptr;
+  const dim3 grid(tile_grid(D.ntiles)), block(kBlock);
+  if (stats && grid.x > st->max_blocks) raise(MLEM_ESTATE, "statistics buffer too small for launch");
+  if (stats)
+    hipLaunchKernelGGL((k_fwd_dict<EPT, VK, true, true>), grid, block, 0, s, m.rowptr, m.csr,
+                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, x, y, out, m.rscale,
+                       st->partials, st->counter, st->results, st->ring);
+  else if (ratio)
+    hipLaunchKernelGGL((k_fwd_dict<EPT, VK, true, false>), grid, block, 0, s, m.rowptr, m.csr,
+                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, x, y, out, m.rscale, nullptr,
+                       nullptr, nullptr, 0u);
+  else
+    hipLaunchKernelGGL((k_fwd_dict<EPT, VK, false, false>), grid, block, 0, s, m.rowptr, m.csr,
+                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, x, y, out, m.rscale, nullptr,
+                       nullptr, nullptr, 0u);
+}
+
 template <int EPT>
 void fwd_d(const DevMatrix& m, const float* x, const int32_t* y, float* out, bool ratio,
            const StatsState* st, hipStream_t s) {
   const DictFormat& D = m.dfwd;
   if (D.ept != EPT || !D.tiles) raise(MLEM_ESTATE, "forward dictionary not prepared");
-  const bool stats = ratio && st != nullptr;
+  switch (D.vk) {
+    case kF32: fwd_dv<EPT, kF32>(m, x, y, out, ratio, st, s); break;
+    case kF16: fwd_dv<EPT, kF16>(m, x, y, out, ratio, st, s); break;
+    default: fwd_dv<EPT, kBF16>(m, x, y, out, ratio, st, s); break;
+  }
+}
+
+template <int EPT, int VK>
+void bwd_dv(const DevMatrix& m, const float* r, const float* norm, float* out, bool update,
+            hipStream_t s) {
+  const DictFormat& D = m.dbwd;
   const dim3 grid(tile_grid(D.ntiles)), block(kBlock);
-  if (stats && grid.x > st->max_blocks) raise(MLEM_ESTATE, "statistics buffer too small for launch");
-  if (stats)
-    hipLaunchKernelGGL((k_fwd_dict<EPT, true, true>), grid, block, 0, s, m.rowptr, m.csr, D.tiles,
-                       D.ntiles, D.dval, D.dlix, D.dict, x, y, out, st->partials, st->counter,
-                       st->results, st->ring);
-  else if (ratio)
-    hipLaunchKernelGGL((k_fwd_dict<EPT, true, false>), grid, block, 0, s, m.rowptr, m.csr, D.tiles,
-                       D.ntiles, D.dval, D.dlix, D.dict, x, y, out, nullptr, nullptr, nullptr, 0u);
+  if (update)
+    hipLaunchKernelGGL((k_bwd_dict<EPT, VK, true>), grid, block, 0, s, m.colptr, m.csc, D.tiles,
+                       D.ntiles, D.dval, D.dlix, D.dict, r, norm, out);
   else
-    hipLaunchKernelGGL((k_fwd_dict<EPT, false, false>), grid, block, 0, s, m.rowptr, m.csr,
-                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, x, y, out, nullptr, nullptr,
-                       nullptr, 0u);
+    hipLaunchKernelGGL((k_bwd_dict<EPT, VK, false>), grid, block, 0, s, m.colptr, m.csc, D.tiles,
+                       D.ntiles, D.dval, D.dlix, D.dict, r, norm, out);
 }
 
 template <int EPT>
@@ -799,13 +924,11 @@ void bwd_d(const DevMatrix& m, const float* r, const float* norm, float* out, bo
            hipStream_t s) {
   const DictFormat& D = m.dbwd;
   if (D.ept != EPT || !D.tiles) raise(MLEM_ESTATE, "backward dictionary not prepared");
-  const dim3 grid(tile_grid(D.ntiles)), block(kBlock);
-  if (update)
-    hipLaunchKernelGGL((k_bwd_dict<EPT, true>), grid, block, 0, s, m.colptr, m.csc, D.tiles,
-                       D.ntiles, D.dval, D.dlix, D.dict, r, norm, out);
-  else
-    hipLaunchKernelGGL((k_bwd_dict<EPT, false>), grid, block, 0, s, m.colptr, m.csc, D.tiles,
-                       D.ntiles, D.dval, D.dlix, D.dict, r, norm, out);
+  switch (D.vk) {
+    case kF32: bwd_dv<EPT, kF32>(m, r, norm, out, update, s); break;
+    case kF16: bwd_dv<EPT, kF16>(m, r, norm, out, update, s); break;
+    default: bwd_dv<EPT, kBF16>(m, r, norm, out, update, s); break;
+  }
 }
 
 // Greedy partition of segments [0, n) into tiles of at most T elements and
@@ -850,7 +973,7 @@ void upload_tiles(const std::vector<uint32_t>& ptr, int ept, uint2*& dev, uint32
 
 // Build the dictionary format of one projection from its device-order
 // segments (host copy of the pointers, device 8-byte records).
-template <int EPT>
+template <int EPT, int VK>
 void build_dict(const std::vector<uint32_t>& ptr, const uint2* el, DictFormat& D, hipStream_t s) {
   constexpr uint32_t T = kBlock * EPT;
   const std::vector<uint32_t> tl = partition_tiles(ptr, T);
@@ -868,9 +991,9 @@ void build_dict(const std::vector<uint32_t>& ptr, const uint2* el, DictFormat& D
   MLEM_HIP(hipMemcpyAsync(t2.p, d2.data(), sizeof(uint2) * d2.size(), hipMemcpyHostToDevice, s));
   std::vector<uint32_t> nd(nt);
   if (nt) {
-    hipLaunchKernelGGL((k_dict_build<EPT>), dim3(nt), dim3(kBlock), 0, s, (const uint2*)el,
+    hipLaunchKernelGGL((k_dict_build<EPT, VK>), dim3(nt), dim3(kBlock), 0, s, (const uint2*)el,
                        (const uint2*)t2.p, (const uint4*)nullptr, (uint32_t*)ndb.p,
-                       (uint32_t*)nullptr, (float*)nullptr, (uint16_t*)nullptr);
+                       (uint32_t*)nullptr, (void*)nullptr, (uint16_t*)nullptr);
     check_launch("dictionary count launch");
     MLEM_HIP(hipMemcpyAsync(nd.data(), ndb.p, 4ull * nt, hipMemcpyDeviceToHost, s));
     MLEM_HIP(hipStreamSynchronize(s));
@@ -888,17 +1011,19 @@ void build_dict(const std::vector<uint32_t>& ptr, const uint2* el, DictFormat& D
     raise(MLEM_EINVAL, "matrix too large for the dictionary format");
   free_dict(D);
   D.ntiles = nt;
+  D.vk = VK;
   D.ndict = std::max<uint64_t>(acc_g, 1);
   D.nval = acc_a + T;  // tail: full-tile loads of the last tile stay in bounds
+  const uint64_t vbytes = (VK == kF32 ? 4ull : 2ull) * D.nval;
   MLEM_HIP(hipMalloc(&D.tiles, sizeof(uint4) * d4.size()));
   MLEM_HIP(hipMalloc(&D.dict, 4ull * D.ndict));
-  MLEM_HIP(hipMalloc(&D.dval, 4ull * D.nval));
+  MLEM_HIP(hipMalloc(&D.dval, vbytes));
   MLEM_HIP(hipMalloc(&D.dlix, 2ull * D.nval));
-  MLEM_HIP(hipMemsetAsync(D.dval, 0, 4ull * D.nval, s));
+  MLEM_HIP(hipMemsetAsync(D.dval, 0, vbytes, s));
   MLEM_HIP(hipMemsetAsync(D.dlix, 0, 2ull * D.nval, s));
   MLEM_HIP(hipMemcpyAsync(D.tiles, d4.data(), sizeof(uint4) * d4.size(), hipMemcpyHostToDevice, s));
   if (nt) {
-    hipLaunchKernelGGL((k_dict_build<EPT>), dim3(nt), dim3(kBlock), 0, s, (const uint2*)el,
+    hipLaunchKernelGGL((k_dict_build<EPT, VK>), dim3(nt), dim3(kBlock), 0, s, (const uint2*)el,
                        (const uint2*)nullptr, (const uint4*)D.tiles, (uint32_t*)nullptr, D.dict,
                        D.dval, D.dlix);
     check_launch("dictionary fill launch");
@@ -907,13 +1032,24 @@ void build_dict(const std::vector<uint32_t>& ptr, const uint2* el, DictFormat& D
   D.ept = EPT;
 }
 
-void ensure_dict(const std::vector<uint32_t>& ptr, const uint2* el, int ept, DictFormat& D,
-                 hipStream_t s) {
-  if (D.ept == ept && D.tiles) return;
+template <int VK>
+void build_dict_vk(const std::vector<uint32_t>& ptr, const uint2* el, int ept, DictFormat& D,
+                   hipStream_t s) {
   switch (ept) {
-    case 8: build_dict<8>(ptr, el, D, s); break;
-    case 16: build_dict<16>(ptr, el, D, s); break;
+    case 8: build_dict<8, VK>(ptr, el, D, s); break;
+    case 16: build_dict<16, VK>(ptr, el, D, s); break;
     default: raise(MLEM_EINVAL, "unsupported dictionary tile size");
+  }
+}
+
+void ensure_dict(const std::vector<uint32_t>& ptr, const uint2* el, int ept, int vk,
+                 DictFormat& D, hipStream_t s) {
+  if (D.ept == ept && D.vk == vk && D.tiles) return;
+  switch (vk) {
+    case kF32: build_dict_vk<kF32>(ptr, el, ept, D, s); break;
+    case kF16: build_dict_vk<kF16>(ptr, el, ept, D, s); break;
+    case kBF16: build_dict_vk<kBF16>(ptr, el, ept, D, s); break;
+    default: raise(MLEM_EINVAL, "unsupported value storage");
   }
 }
 
@@ -944,8 +1080,8 @@ void prepare_tiles(DevMatrix& m, hipStream_t s) {
     upload_tiles(m.h_rowptr, m.fwd.ept, m.tiles_fwd, m.ntiles_fwd, m.tiles_fwd_ept, s);
   if (m.bwd.kind == kTiles && m.csc)
     upload_tiles(m.h_colptr, m.bwd.ept, m.tiles_bwd, m.ntiles_bwd, m.tiles_bwd_ept, s);
-  if (m.fwd.kind == kDict) ensure_dict(m.h_rowptr, m.csr, m.fwd.ept, m.dfwd, s);
-  if (m.bwd.kind == kDict && m.csc) ensure_dict(m.h_colptr, m.csc, m.bwd.ept, m.dbwd, s);
+  if (m.fwd.kind == kDict) ensure_dict(m.h_rowptr, m.csr, m.fwd.ept, m.vk, m.dfwd, s);
+  if (m.bwd.kind == kDict && m.csc) ensure_dict(m.h_colptr, m.csc, m.bwd.ept, m.vk, m.dbwd, s);
 }
 
 unsigned stats_max_blocks(const DevMatrix& m) {
@@ -1052,6 +1188,32 @@ void launch_colsums_atomic(const DevMatrix& m, float* norm, hipStream_t s) {
   hipLaunchKernelGGL(k_colsum_atomic, dim3(std::min<uint32_t>(blocks_for(m.nnz), 8 * m.cus)),
                      dim3(kBlock), 0, s, m.csr, m.nnz, norm);
   check_launch("column sums launch");
+}
+
+void launch_quantize(DevMatrix& m, int vk, float* rscale, hipStream_t s) {
+  if (m.rows == 0) return;
+  const dim3 grid(blocks_for((uint64_t)m.rows * 16)), block(kBlock);
+  if (vk == kF16)
+    hipLaunchKernelGGL(k_quantize_rows<kF16>, grid, block, 0, s, m.rowptr, m.csr, m.rows, rscale);
+  else if (vk == kBF16)
+    hipLaunchKernelGGL(k_quantize_rows<kBF16>, grid, block, 0, s, m.rowptr, m.csr, m.rows, rscale);
+  else
+    raise(MLEM_EINVAL, "unsupported value storage");
+  check_launch("quantization launch");
+}
+
+void launch_colsums_scaled(const DevMatrix& m, float* norm, hipStream_t s) {
+  if (!m.csc || !m.rscale) raise(MLEM_ESTATE, "scaled column sums need the CSC and row scales");
+  hipLaunchKernelGGL(k_colsum_scaled, dim3(blocks_for(m.cols)), dim3(kBlock), 0, s, m.colptr,
+                     m.csc, m.rscale, norm, m.cols);
+  check_launch("column sums launch");
+}
+
+void launch_scale_rows(const DevMatrix& m, float* r, hipStream_t s) {
+  if (!m.rscale || m.rows == 0) return;
+  hipLaunchKernelGGL(k_scale_rows, dim3(blocks_for(m.rows)), dim3(kBlock), 0, s, r, m.rscale,
+                     m.rows);
+  check_launch("row scale launch");
 }
 
 void launch_fill(float* p, float v, size_t n, hipStream_t s) {

@@ -251,41 +251,61 @@ int orc_omp_mlem(uint32_t rows, uint32_t cols, const uint64_t* rowidx,
 }
 
 /* ---------------------------------------------------------------- FP64 -- */
+/* a_ij = value, or (double) rscale[i] * value when a row scale is given
+ * (reduced-precision storage: the value is the 16-bit quotient q_ij). The
+ * product of an fp32 scale and a 16-bit quotient is exact in double. */
+#define A64(rscale, i, e) ((rscale) ? (double)(rscale)[i] * (double)(e).value : (double)(e).value)
 
-void orc_f64_column_sums(uint32_t rows, uint32_t cols, const uint64_t* rowidx,
-                         const orc_elem* el, double* norm) {
+static void f64_column_sums(uint32_t rows, uint32_t cols, const uint64_t* rowidx,
+                            const orc_elem* el, const float* rscale, double* norm) {
   for (uint32_t j = 0; j < cols; ++j) norm[j] = 0.0;
   for (uint32_t r = 0; r < rows; ++r)
     for (uint64_t k = ROW_BEGIN(rowidx, r); k < rowidx[r]; ++k)
-      norm[el[k].column] += (double)el[k].value;
+      norm[el[k].column] += A64(rscale, r, el[k]);
 }
 
-void orc_f64_fwproj(uint32_t rows, const uint64_t* rowidx, const orc_elem* el,
-                    const double* x, double* f) {
+static void f64_fwproj(uint32_t rows, const uint64_t* rowidx, const orc_elem* el,
+                       const float* rscale, const double* x, double* f) {
   for (uint32_t r = 0; r < rows; ++r) {
     double res = 0.0;
     for (uint64_t k = ROW_BEGIN(rowidx, r); k < rowidx[r]; ++k)
-      res += (double)el[k].value * x[el[k].column];
+      res += A64(rscale, r, el[k]) * x[el[k].column];
     f[r] = res;
   }
 }
 
-void orc_f64_bkproj(uint32_t rows, uint32_t cols, const uint64_t* rowidx,
-                    const orc_elem* el, const double* r, double* u) {
+static void f64_bkproj(uint32_t rows, uint32_t cols, const uint64_t* rowidx,
+                       const orc_elem* el, const float* rscale, const double* r, double* u) {
   for (uint32_t j = 0; j < cols; ++j) u[j] = 0.0;
   for (uint32_t i = 0; i < rows; ++i)
     for (uint64_t k = ROW_BEGIN(rowidx, i); k < rowidx[i]; ++k)
-      u[el[k].column] += (double)el[k].value * r[i];
+      u[el[k].column] += A64(rscale, i, el[k]) * r[i];
 }
 
-/* The MLEM iteration in double. y holds the counts as doubles (int32 counts,
- * or relaxed real counts). init_given = 0: x0 = sum(y)/sum(norm).
+void orc_f64_column_sums(uint32_t rows, uint32_t cols, const uint64_t* rowidx,
+                         const orc_elem* el, double* norm) {
+  f64_column_sums(rows, cols, rowidx, el, NULL, norm);
+}
+
+void orc_f64_fwproj(uint32_t rows, const uint64_t* rowidx, const orc_elem* el,
+                    const double* x, double* f) {
+  f64_fwproj(rows, rowidx, el, NULL, x, f);
+}
+
+void orc_f64_bkproj(uint32_t rows, uint32_t cols, const uint64_t* rowidx,
+                    const orc_elem* el, const double* r, double* u) {
+  f64_bkproj(rows, cols, rowidx, el, NULL, r, u);
+}
+
+/* The MLEM iteration in double on a_ij = rscale[i] * value (rscale may be
+ * NULL: a_ij = value). y holds the counts as doubles (int32 counts, or
+ * relaxed real counts). init_given = 0: x0 = sum(y)/sum(norm).
  * Per-iteration outputs (may be NULL): Poisson log-likelihood of the iterate
  * entering iteration k, L = sum_{i: f_i != 0} (y_i log f_i - f_i), and
  * sum_i f_i. Returns 0 or -4. */
-int orc_f64_mlem(uint32_t rows, uint32_t cols, const uint64_t* rowidx,
-                 const orc_elem* el, const double* y, double* x, int iters,
-                 int init_given, double* loglik, double* fwd_total) {
+int orc_f64_mlem_scaled(uint32_t rows, uint32_t cols, const uint64_t* rowidx,
+                        const orc_elem* el, const float* rscale, const double* y, double* x,
+                        int iters, int init_given, double* loglik, double* fwd_total) {
   double* norm = (double*)malloc(sizeof(double) * (cols ? cols : 1));
   double* f = (double*)malloc(sizeof(double) * (rows ? rows : 1));
   double* u = (double*)malloc(sizeof(double) * (cols ? cols : 1));
@@ -293,7 +313,7 @@ int orc_f64_mlem(uint32_t rows, uint32_t cols, const uint64_t* rowidx,
     free(norm); free(f); free(u);
     return -4;
   }
-  orc_f64_column_sums(rows, cols, rowidx, el, norm);
+  f64_column_sums(rows, cols, rowidx, el, rscale, norm);
   if (!init_given) {
     double sn = 0.0, sy = 0.0;
     for (uint32_t j = 0; j < cols; ++j) sn += norm[j];
@@ -301,7 +321,7 @@ int orc_f64_mlem(uint32_t rows, uint32_t cols, const uint64_t* rowidx,
     for (uint32_t j = 0; j < cols; ++j) x[j] = sy / sn;
   }
   for (int it = 0; it < iters; ++it) {
-    orc_f64_fwproj(rows, rowidx, el, x, f);
+    f64_fwproj(rows, rowidx, el, rscale, x, f);
     double ll = 0.0, tot = 0.0;
     for (uint32_t i = 0; i < rows; ++i) {
       tot += f[i];
@@ -312,10 +332,65 @@ int orc_f64_mlem(uint32_t rows, uint32_t cols, const uint64_t* rowidx,
     }
     if (loglik) loglik[it] = ll;
     if (fwd_total) fwd_total[it] = tot;
-    orc_f64_bkproj(rows, cols, rowidx, el, f, u);
+    f64_bkproj(rows, cols, rowidx, el, rscale, f, u);
     for (uint32_t j = 0; j < cols; ++j)
       x[j] *= (norm[j] != 0.0) ? (u[j] / norm[j]) : u[j];
   }
   free(norm); free(f); free(u);
   return 0;
+}
+
+int orc_f64_mlem(uint32_t rows, uint32_t cols, const uint64_t* rowidx,
+                 const orc_elem* el, const double* y, double* x, int iters,
+                 int init_given, double* loglik, double* fwd_total) {
+  return orc_f64_mlem_scaled(rows, cols, rowidx, el, NULL, y, x, iters, init_given, loglik,
+                             fwd_total);
+}
+
+/* ------------------------------------------------- reduced precision -- */
+/* Round an fp32 value to the nearest fp16 value (ties to even, subnormals
+ * kept, no overflow handling: |v| <= 1 here), returned as fp32. Works on the
+ * binary representation: keep 11 significant bits (fewer in the subnormal
+ * range, where the fp16 quantum is 2^-24). */
+static float round_fp16(float v) {
+  if (v == 0.0f) return v;
+  int e;
+  const double m = frexp((double)v, &e);          /* v = m 2^e, 0.5 <= |m| < 1 */
+  int bits = 11;                                  /* fp16: 10 stored + hidden  */
+  if (e - 1 < -14) bits = 11 - (-14 - (e - 1));   /* subnormal: quantum 2^-24  */
+  if (bits < 0) return 0.0f * v;
+  const double scaled = ldexp(m, bits);           /* integer part = kept bits  */
+  double r = nearbyint(scaled);                   /* default mode: ties even   */
+  return (float)ldexp(r, e - bits);
+}
+
+/* Round to the nearest bf16 value (ties to even): keep 8 significant bits
+ * of the fp32 significand. */
+static float round_bf16(float v) {
+  if (v == 0.0f) return v;
+  int e;
+  const double m = frexp((double)v, &e);
+  return (float)ldexp(nearbyint(ldexp(m, 8)), e - 8);
+}
+
+/* The reduced-precision matrix: s_i = max_k |a_ik| (1 for an all-zero row),
+ * q_ik = round(a_ik / s_i) with the quotient rounded to fp32 first (fp32
+ * division, flush-to-zero), then to fp16 (kind 1) or bf16 (kind 2). Writes
+ * out[k] = {column, q_ik} and rscale[i] = s_i. */
+void orc_quantize(uint32_t rows, const uint64_t* rowidx, const orc_elem* el, int kind,
+                  orc_elem* out, float* rscale) {
+  const unsigned old = ftz_on();
+  for (uint32_t r = 0; r < rows; ++r) {
+    float mx = 0.0f;
+    for (uint64_t k = ROW_BEGIN(rowidx, r); k < rowidx[r]; ++k)
+      mx = fabsf(el[k].value) > mx ? fabsf(el[k].value) : mx;
+    const float s = mx > 0.0f ? mx : 1.0f;
+    rscale[r] = s;
+    for (uint64_t k = ROW_BEGIN(rowidx, r); k < rowidx[r]; ++k) {
+      volatile float qf = el[k].value / s;
+      out[k].column = el[k].column;
+      out[k].value = kind == 1 ? round_fp16(qf) : round_bf16(qf);
+    }
+  }
+  ftz_restore(old);
 }

@@ -46,6 +46,9 @@ def lib() -> C.CDLL:
             "orc_f64_fwproj": (None, [_u32, _P, _P, _P, _P]),
             "orc_f64_bkproj": (None, [_u32, _u32, _P, _P, _P, _P]),
             "orc_f64_mlem": (C.c_int, [_u32, _u32, _P, _P, _P, _P, C.c_int, C.c_int, _P, _P]),
+            "orc_f64_mlem_scaled": (C.c_int, [_u32, _u32, _P, _P, _P, _P, _P, C.c_int, C.c_int,
+                                              _P, _P]),
+            "orc_quantize": (None, [_u32, _P, _P, C.c_int, _P, _P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -176,6 +179,33 @@ def f64_mlem(rowidx, elements, cols, y, iters, x0=None):
     ll, ft = np.zeros(n), np.zeros(n)
     rc = lib().orc_f64_mlem(rowidx.size, cols, _p(rowidx), _p(el), _p(y), _p(x), iters,
                             0 if x0 is None else 1, _p(ll), _p(ft))
+    if rc:
+        raise MemoryError("oracle allocation failed")
+    return x, ll[:iters], ft[:iters]
+
+
+QUANT_KINDS = {"fp16": 1, "bf16": 2}
+
+
+def quantize(rowidx, elements, kind: str):
+    """Reduced-precision matrix: (elements with q_ij values, rscale[rows])."""
+    rowidx, el = _mat(rowidx, elements)
+    out = np.empty_like(el)
+    s = np.empty(rowidx.size, np.float32)
+    lib().orc_quantize(rowidx.size, _p(rowidx), _p(el), QUANT_KINDS[kind], _p(out), _p(s))
+    return out, s
+
+
+def f64_mlem_scaled(rowidx, elements, rscale, cols, y, iters, x0=None):
+    """FP64 MLEM on a_ij = rscale[i] * value_ij (see f64_mlem)."""
+    rowidx, el = _mat(rowidx, elements)
+    rscale = np.ascontiguousarray(rscale, np.float32)
+    y = np.ascontiguousarray(y, np.float64)
+    x = np.zeros(cols, np.float64) if x0 is None else np.array(x0, np.float64, copy=True)
+    n = max(iters, 1)
+    ll, ft = np.zeros(n), np.zeros(n)
+    rc = lib().orc_f64_mlem_scaled(rowidx.size, cols, _p(rowidx), _p(el), _p(rscale), _p(y),
+                                   _p(x), iters, 0 if x0 is None else 1, _p(ll), _p(ft))
     if rc:
         raise MemoryError("oracle allocation failed")
     return x, ll[:iters], ft[:iters]

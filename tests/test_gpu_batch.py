@@ -1,0 +1,129 @@
+"""Batched (many-sinogram) reconstruction on the GPU against the oracle.
+
+Every frame's image after k iterations must be bit-identical to k iterations of
+the fp32 restatement of seqcsr4mlem; the per-frame FP64 log-likelihood history
+must match the FP64 oracle's.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import rel_linf
+import mlem_amd as M
+from mlem_amd import synth
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def make_frames(p, frames, lo=2e4, hi=2e6):
+    """Per-frame sinograms: the workload's phantom with per-ellipsoid scales
+    and log-uniform total counts."""
+    w = synth.WORKLOADS[p.name]
+    ys = []
+    counts = synth.frame_counts(frames, lo, hi)
+    for f, sc in enumerate(synth.frame_scales(frames, w["ellipsoids"])):
+        x = synth.make_phantom(w["grid"], w["ellipsoids"], 2000 + w["key"], w["semi"], scale=sc)
+        ybar = synth.project(p.matrix, x)
+        ys.append(synth.poisson(ybar, counts[f], 5000 + f))
+    return np.stack(ys)
+
+
+@pytest.fixture(scope="module")
+def tiny_batch(tiny_problem):
+    A = M.Csr4Matrix.from_arrays(tiny_problem.matrix.rows, tiny_problem.matrix.cols,
+                                 tiny_problem.matrix.rowidx, tiny_problem.matrix.elements,
+                                 grid=tiny_problem.matrix.grid)
+    Y = make_frames(tiny_problem, 11)
+    yield tiny_problem, A, Y
+    A.close()
+
+
+@pytest.mark.parametrize("width", [4, 8, 32])
+def test_batch_budgets_bitwise(tiny_batch, width):
+    p, A, Y = tiny_batch
+    m = p.matrix
+    budgets = np.array([1, 2, 3, 5, 8, 13, 4, 0, 7, 9, 6], np.int32)
+    B = M.Batch(A, Y, budgets, block_width=width)
+    n = B.run(100)
+    assert n == budgets.max()
+    prog = B.progress()
+    assert prog["active"] == 0 and prog["frame_iterations"] == budgets.sum()
+    for f in range(Y.shape[0]):
+        it, ll = B.frame_info(f)
+        assert it == budgets[f]
+        xo = orc.mlem(m.rowidx, m.elements, m.cols, Y[f], int(budgets[f]))[0]
+        assert np.array_equal(B.image(f), xo), f
+        if it:
+            _, ll64, _ = orc.f64_mlem(m.rowidx, m.elements, m.cols, Y[f].astype(np.float64), it)
+            assert rel_linf(ll, ll64) < 1e-6
+    B.close()
+
+
+def test_batch_early_stop(tiny_batch):
+    p, A, Y = tiny_batch
+    m = p.matrix
+    tol = 2e-4
+    B = M.Batch(A, Y, np.full(Y.shape[0], 60, np.int32), rel_tol=tol, block_width=8)
+    B.run(1000)
+    for f in range(Y.shape[0]):
+        it, ll = B.frame_info(f)
+        assert 2 <= it <= 60
+        rel = np.abs(np.diff(ll)) / np.abs(ll[1:])
+        if it < 60:
+            assert rel[-1] <= tol and np.all(rel[:-1] > tol)   # stopped at the first small change
+        xo = orc.mlem(m.rowidx, m.elements, m.cols, Y[f], it)[0]
+        assert np.array_equal(B.image(f), xo), f
+    B.close()
+
+
+def test_batch_graph_matches_direct(tiny_batch):
+    p, A, Y = tiny_batch
+    budgets = np.array([3, 6, 2, 5, 4, 1, 6, 6, 3, 2, 5], np.int32)
+    a = M.Batch(A, Y, budgets, block_width=4, use_graph=True)
+    b = M.Batch(A, Y, budgets, block_width=4, use_graph=False)
+    a.run(10)
+    b.run(10)
+    for f in range(Y.shape[0]):
+        assert np.array_equal(a.image(f), b.image(f))
+    a.close()
+    b.close()
+
+
+def test_batch_incremental_runs(tiny_batch):
+    p, A, Y = tiny_batch
+    m = p.matrix
+    B = M.Batch(A, Y[:5], None, block_width=4)
+    assert B.run(3) == 3
+    assert B.run(4) == 4
+    for f in range(5):
+        assert B.frame_info(f)[0] == 7
+        assert np.array_equal(B.image(f), orc.mlem(m.rowidx, m.elements, m.cols, Y[f], 7)[0])
+    B.close()
+
+
+def test_batch_small_workload():
+    # 32^3 grid, 262,144 lines: 6 frames, budgets up to 12, width 4 (two blocks)
+    p = synth.make_problem("small")
+    m = p.matrix
+    A = M.Csr4Matrix.from_arrays(m.rows, m.cols, m.rowidx, m.elements, grid=m.grid)
+    Y = make_frames(p, 6, 1e5, 1e7)
+    budgets = np.array([12, 3, 7, 1, 10, 5], np.int32)
+    B = M.Batch(A, Y, budgets, block_width=4)
+    B.run(50)
+    for f in range(6):
+        xo = orc.mlem(m.rowidx, m.elements, m.cols, Y[f], int(budgets[f]))[0]
+        assert np.array_equal(B.image(f), xo), f
+    B.close()
+    A.close()
+
+
+def test_batch_argument_errors(tiny_batch):
+    p, A, Y = tiny_batch
+    with pytest.raises(ValueError):
+        M.Batch(A, Y[:, :-1])
+    with pytest.raises(M.MlemError):
+        M.Batch(A, Y, block_width=12)
+    with pytest.raises(M.MlemError):
+        M.Batch(A, Y, np.full(Y.shape[0], -1, np.int32))
