@@ -30,12 +30,49 @@ PEAK_HBM_GBS = 8000.0
 METRIC = "MLEM iter/s + HBM GB/s (~200M nnz fp32); batched 64-frame vs plain speedup"
 
 
-def kernel_bytes(rows: int, cols: int, nnz: int) -> dict:
+def kernel_bytes(rows: int, cols: int, nnz: int, precision: str = "fp32") -> dict:
     """Algorithmic HBM bytes per launch: every matrix record, index and vector
-    element counted once (u32 row/column pointers)."""
-    fwd = 8 * nnz + 4 * (rows + 1) + 4 * cols + 4 * rows + 4 * rows
-    bwd = 8 * nnz + 4 * (cols + 1) + 4 * rows + 4 * cols + 8 * cols
+    element counted once (u32 row/column pointers). Records are 8 bytes
+    (u32 index + fp32 value) or 6 bytes with 16-bit values, plus the fp32 row
+    scale in the forward pass."""
+    rec = 8 if precision == "fp32" else 6
+    scale = 0 if precision == "fp32" else 4 * rows
+    fwd = rec * nnz + 4 * (rows + 1) + 4 * cols + 4 * rows + 4 * rows + scale
+    bwd = rec * nnz + 4 * (cols + 1) + 4 * rows + 4 * cols + 8 * cols
     return {"fwd": fwd, "bwd": bwd, "iteration": fwd + bwd}
+
+
+def fp64_check(A, prob, precision: str, iters: int) -> dict:
+    """`iters` GPU iterations against FP64 MLEM on the original values (the
+    accuracy of the stored precision) and, for 16-bit storage, on the
+    dequantized values s_i q_ij (parity of the kernels)."""
+    import mlem_amd as M
+    from oracle import oracle as orc  # accuracy check only
+    m = prob.matrix
+    R = M.Recon(A, prob.y, mode="fast")
+    R.run(iters)
+    x = R.image().astype(np.float64)
+    R.close()
+    y = prob.y.astype(np.float64)
+    t0 = time.time()
+    x64 = orc.f64_mlem_par(m.rowidx, m.elements, m.cols, y, iters)[0]
+    out = {"iterations": iters, "threads": os.cpu_count()}
+
+    def err(a, b):
+        floor = 1e-6 * float(np.max(np.abs(b)))
+        return {"rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+                "rel_linf": float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))}
+
+    out["vs_fp64_original"] = err(x, x64)
+    print(f"[bench] fp64 oracle on the original values: {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    if precision != "fp32":
+        q, s = orc.quantize(m.rowidx, m.elements, precision)
+        xq = orc.f64_mlem_par(m.rowidx, q, m.cols, y, iters, rscale=s)[0]
+        out["vs_fp64_dequantized"] = err(x, xq)
+        out["fp64_dequantized_vs_original"] = err(xq, x64)
+    out["oracle_s"] = time.time() - t0
+    return out
 
 
 def cpu_baseline(problem, iters: int, parity_x: np.ndarray | None):
@@ -121,7 +158,14 @@ def main() -> int:
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="first run the plain copy kernel over this many MiB (calibrates "
                          "hardware byte counters under rocprofv3 --pmc)")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16"],
+                    help="matrix value storage (16-bit: per-row fp32 scale)")
+    ap.add_argument("--fp64-iters", type=int, default=0,
+                    help="also compare this many iterations with FP64 MLEM (oracle, all "
+                         "host cores); 0 = skip")
     args = ap.parse_args()
+    if args.precision != "fp32" and (args.mode == "atomic" or args.batched_frames):
+        args.batched_frames = 0  # the batched path runs on fp32 values only
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU row sharding is a design note, not built)", file=sys.stderr)
@@ -143,7 +187,8 @@ def main() -> int:
           f"(generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     t0 = time.time()
     A = M.Csr4Matrix.from_arrays(m.rows, m.cols, m.rowidx, m.elements, grid=m.grid,
-                                 no_csc=(args.mode == "atomic"), keep_order=args.keep_order)
+                                 no_csc=(args.mode == "atomic" and args.precision == "fp32"),
+                                 keep_order=args.keep_order, precision=args.precision)
     print(f"[bench] matrix in HBM + CSC in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     if args.launch:
         f, fp, b, bp = args.launch.split(",")
@@ -164,22 +209,25 @@ def main() -> int:
     ms_per_step = 1000.0 * wall / args.steps
     value = args.steps / wall
 
-    B = kernel_bytes(m.rows, m.cols, m.nnz)
+    B = kernel_bytes(m.rows, m.cols, m.nnz, args.precision)
     fwd_ms, bwd_ms = float(np.mean(kf)), float(np.mean(kb))
     dom = "fwd" if fwd_ms >= bwd_ms else "bwd"
     dom_ms = max(fwd_ms, bwd_ms)
     achieved = B[dom] / (dom_ms * 1e-3) / 1e9
-    name = {"fwd": "k_fwd (CSR forward projection + fused ratio)",
-            "bwd": "k_bwd (CSC gather backprojection + fused update)"}[dom]
-    if args.mode == "atomic":
-        name = {"fwd": "k_fwd (CSR forward projection + fused ratio)",
-                "bwd": "k_scatter + k_update (atomic backprojection)"}[dom]
+    suffix = {"dict": "_dict", "tiles": "_tiles", "groups": ""}[launch[dom][0]]
+    if args.mode == "exact":
+        suffix = ""
+    name = {"fwd": f"k_fwd{suffix} (CSR forward projection + fused ratio)",
+            "bwd": f"k_bwd{suffix} (CSC gather backprojection + fused update)"}[dom]
+    if args.mode == "atomic" and dom == "bwd":
+        name = "k_scatter + k_update (atomic backprojection)"
 
     # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
     # FETCH_SIZE/WRITE_SIZE passes (copy-kernel calibrated) of this workload
     # and launch configuration.
     traffic = None
-    tfile = ROOT / "profiles" / f"traffic_{args.workload}_{args.mode}.json"
+    ptag = "" if args.precision == "fp32" else f"_{args.precision}"
+    tfile = ROOT / "profiles" / f"traffic_{args.workload}_{args.mode}{ptag}.json"
     if tfile.exists():
         t = json.loads(tfile.read_text())
         if t.get("launch") == json.loads(json.dumps(launch)):
@@ -204,6 +252,11 @@ def main() -> int:
             batched = {"error": f"{type(e).__name__}: {e}"}
         print(f"[bench] batched: {batched}", file=sys.stderr, flush=True)
 
+    accuracy = None
+    if args.fp64_iters > 0:
+        accuracy = fp64_check(A, prob, args.precision, args.fp64_iters)
+        print(f"[bench] fp64 check: {accuracy}", file=sys.stderr, flush=True)
+
     line = {
         "metric": METRIC,
         "value": value,
@@ -215,13 +268,14 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if args.precision == "fp32" else f"{args.precision} values, f32 accumulate",
         "data": "synthetic",
         "config": {
-            "workload": f"{args.workload}: plain fp32 MLEM, {'x'.join(map(str, m.grid))} grid, "
-                        f"{m.rows} lines of response, {m.nnz} nonzeros, {m.cols} voxels",
+            "workload": f"{args.workload}: plain MLEM, {args.precision} matrix values, "
+                        f"{'x'.join(map(str, m.grid))} grid, {m.rows} lines of response, "
+                        f"{m.nnz} nonzeros, {m.cols} voxels",
             "rows": m.rows, "cols": m.cols, "nnz": m.nnz, "mode": args.mode,
-            "launch": launch, "iterations_run": done,
+            "precision": args.precision, "launch": launch, "iterations_run": done,
         },
         "hbm_gbs_iteration": B["iteration"] / (ms_per_step * 1e-3) / 1e9,
         "kernel_ms": {"fwd": fwd_ms, "bwd": bwd_ms, "span_per_iter": span / args.steps},
@@ -240,6 +294,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "parity": parity,
         "batched": batched,
+        "fp64_check": accuracy,
     }
     print(json.dumps(line), flush=True)
     R.close()

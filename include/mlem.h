@@ -247,6 +247,33 @@ MLEM_API int mlem_batch_progress(const mlem_batch* b, uint32_t* active_frames,
                                  uint32_t* blocks);
 MLEM_API void mlem_batch_destroy(mlem_batch* b);
 
+/* ---- differentiable unrolled MLEM ----------------------------------------
+ * x_K = K iterations of mlem() (seqcsr4mlem.cpp:137-189) from given initial
+ * images, with real-valued counts y, as a function of (y, x0); reverse mode
+ * through the multiplicative update gives dL/dy and dL/dx0 from dL/dx_K.
+ * The frames of a batch are the right-hand sides of one SpMM block. The tape
+ * keeps the iterate every checkpoint_interval iterations (0: round(sqrt K));
+ * mlem_backward_k recomputes each segment once from its checkpoint. Where
+ * f_i = 0 the ratio is 0 and its derivative is taken as 0. No reference
+ * counterpart (the reference has no gradients). */
+typedef struct mlem_tape mlem_tape;
+/* y: frames*rows float counts; x0: frames*cols initial images (NULL: per
+ * frame (float)(sum y / sum n), then treated as a constant); xK: frames*cols
+ * output (may be NULL); tape: receives the tape for mlem_backward_k (NULL:
+ * forward only). All arrays frame-major, file row / voxel order. */
+MLEM_API int mlem_forward_k(mlem_matrix* m, uint32_t frames, const float* y, uint32_t rows,
+                            const float* x0, uint32_t cols, int K, int checkpoint_interval,
+                            float* xK, mlem_tape** tape);
+/* grad_xK: frames*cols dL/dx_K; grad_y (frames*rows) and grad_x0
+ * (frames*cols) may be NULL. May be called more than once per tape. */
+MLEM_API int mlem_backward_k(mlem_tape* t, const float* grad_xK, float* grad_y,
+                             float* grad_x0);
+MLEM_API int mlem_tape_info(const mlem_tape* t, int* block_width, int* checkpoint_interval,
+                            int* checkpoints, int* recomputed_iterations,
+                            uint64_t* spmm_launches);
+/* Destroy a tape (before closing its matrix). */
+MLEM_API void mlem_tape_destroy(mlem_tape* t);
+
 /* ---- launch configuration (autotuner) --------------------------------------
  * FAST-mode kernel family and its parameter, per projection:
  *   MLEM_KERNEL_TILES  param = elements per lane (4, 8 or 16): 256-lane

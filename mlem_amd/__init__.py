@@ -370,6 +370,65 @@ class Batch:
             pass
 
 
+class Unrolled:
+    """K unrolled MLEM iterations as a differentiable function of the counts
+    y (real-valued, [frames, rows]) and the initial images x0 ([frames, cols];
+    None: per-frame uniform sum(y)/sum(n), held constant).
+
+    u = Unrolled(A, y, x0, K)       runs the forward pass; u.xK is x_K
+    gy, gx0 = u.backward(dL/dx_K)   reverse mode (recomputes from checkpoints)
+    """
+
+    def __init__(self, matrix: Csr4Matrix, y, x0=None, K: int = 20, *,
+                 checkpoint_interval: int = 0, keep_tape: bool = True):
+        y = np.ascontiguousarray(np.atleast_2d(y), np.float32)
+        frames, rows = y.shape
+        if rows != matrix.rows():
+            raise ValueError("y must be [frames, rows]")
+        cols = matrix.columns()
+        if x0 is not None:
+            x0 = np.ascontiguousarray(np.atleast_2d(x0), np.float32)
+            if x0.shape != (frames, cols):
+                raise ValueError("x0 must be [frames, cols]")
+        self.matrix, self.frames, self.K = matrix, frames, int(K)
+        self.xK = np.zeros((frames, cols), np.float32)
+        t = C.c_void_p()
+        check(lib().mlem_forward_k(matrix.handle, frames, _ptr(y), rows, _ptr(x0), cols,
+                                   self.K, int(checkpoint_interval), _ptr(self.xK),
+                                   C.byref(t) if keep_tape else None))
+        self._t = t if keep_tape else None
+
+    def backward(self, grad_xK):
+        g = np.ascontiguousarray(np.atleast_2d(grad_xK), np.float32)
+        if g.shape != self.xK.shape:
+            raise ValueError("grad_xK must be [frames, cols]")
+        if self._t is None:
+            raise RuntimeError("forward pass ran without a tape")
+        gy = np.zeros((self.frames, self.matrix.rows()), np.float32)
+        gx0 = np.zeros_like(self.xK)
+        check(lib().mlem_backward_k(self._t, _ptr(g), _ptr(gy), _ptr(gx0)))
+        return gy, gx0
+
+    def info(self) -> dict:
+        v = [C.c_int() for _ in range(4)]
+        n = C.c_uint64()
+        check(lib().mlem_tape_info(self._t, *[C.byref(a) for a in v], C.byref(n)))
+        return {"block_width": v[0].value, "checkpoint_interval": v[1].value,
+                "checkpoints": v[2].value, "recomputed_iterations": v[3].value,
+                "spmm_launches": n.value}
+
+    def close(self) -> None:
+        if getattr(self, "_t", None) is not None:
+            lib().mlem_tape_destroy(self._t)
+            self._t = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def calibration_copy(nbytes: int, repeats: int = 5) -> float:
     ms = C.c_float()
     check(lib().mlem_calibration_copy(int(nbytes), int(repeats), C.byref(ms)))

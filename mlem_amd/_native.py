@@ -103,6 +103,12 @@ SIGNATURES = {
     "mlem_batch_progress": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
                                       C.POINTER(C.c_int), C.POINTER(C.c_uint32)]),
     "mlem_batch_destroy": (None, [_P]),
+    "mlem_forward_k": (C.c_int, [_P, _u32, _P, _u32, _P, _u32, C.c_int, C.c_int, _P,
+                                 C.POINTER(_P)]),
+    "mlem_backward_k": (C.c_int, [_P, _P, _P, _P]),
+    "mlem_tape_info": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),
+    "mlem_tape_destroy": (None, [_P]),
     "mlem_calibration_copy": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_float)]),
     "mlem_device_sync": (C.c_int, []),
 }

@@ -25,16 +25,15 @@
 
 #include "internal.hpp"
 #include "mlem.h"
+#include "spmm.hpp"
 
 using namespace mlem;
 
 namespace {
 
-constexpr int kBlock = 256;
-constexpr uint32_t kTile = 2048;  // records per tile (16 KB of LDS)
+using spmm::kBlock;
+using spmm::kTile;
 constexpr unsigned kRing = 1024;  // per-iteration statistics ring
-
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
 inline uint32_t blocks_for(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, (n + kBlock - 1) / kBlock); }
 
@@ -100,99 +99,57 @@ __device__ void batch_stats_finish(double ll, double tot, double2* __restrict__ 
   }
 }
 
-// Sequential fp32 dot product of one segment's staged records with W
-// interleaved right-hand sides (lane f: frame f), multiply and add rounded
-// separately, four gathers in flight.
+// Forward epilogue: ratio r = (f != 0) ? y/f : 0 and the frame's FP64
+// statistics.
 template <int W>
-__device__ __forceinline__ float seg_dot_batch(const uint2* rec, uint32_t b, uint32_t e,
-                                               const float* __restrict__ v, uint32_t f) {
-  float acc = 0.f;
-  uint32_t k = b;
-  for (; k + 4 <= e; k += 4) {
-    const uint2 r0 = rec[k], r1 = rec[k + 1], r2 = rec[k + 2], r3 = rec[k + 3];
-    const float g0 = v[(size_t)r0.x * W + f], g1 = v[(size_t)r1.x * W + f];
-    const float g2 = v[(size_t)r2.x * W + f], g3 = v[(size_t)r3.x * W + f];
-    acc = acc + __uint_as_float(r0.y) * g0;
-    acc = acc + __uint_as_float(r1.y) * g1;
-    acc = acc + __uint_as_float(r2.y) * g2;
-    acc = acc + __uint_as_float(r3.y) * g3;
+struct FwdRatio {
+  const int32_t* __restrict__ Y;
+  float* __restrict__ R;
+  double ll = 0.0, tot = 0.0;
+  __device__ void operator()(uint32_t row, uint32_t f, float F) {
+    const size_t i = (size_t)row * W + f;
+    const int32_t yi = Y[i];
+    R[i] = (F != 0.f) ? (float)yi / F : 0.f;
+    tot += (double)F;
+    if (F > 0.f) ll += (yi != 0 ? (double)yi * log((double)F) : 0.0) - (double)F;
   }
-  for (; k < e; ++k) {
-    const uint2 r = rec[k];
-    acc = acc + __uint_as_float(r.y) * v[(size_t)r.x * W + f];
+};
+
+// Backward epilogue: x *= (n != 0) ? u/n : u for live lanes (bit set in
+// the block's mask); finished frames keep their image.
+template <int W>
+struct BwdUpdate {
+  const float* __restrict__ norm;
+  float* __restrict__ X;
+  uint32_t mask;
+  __device__ void operator()(uint32_t col, uint32_t f, float u) {
+    if (!((mask >> f) & 1u)) return;
+    const float n = norm[col];
+    const size_t i = (size_t)col * W + f;
+    X[i] = X[i] * ((n != 0.f) ? u / n : u);
   }
-  return acc;
-}
+};
 
 // Forward projection + ratio (+ per-frame statistics) for one RHS block.
-// Persistent grid over the row tiles.
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_fwd_batch(
     const uint32_t* __restrict__ rowptr, const uint2* __restrict__ csr,
     const uint2* __restrict__ tiles, uint32_t ntiles, const float* __restrict__ X,
     const int32_t* __restrict__ Y, float* __restrict__ R, double2* __restrict__ partials,
     unsigned* __restrict__ counter, double2* __restrict__ results, unsigned ring) {
-  constexpr int G = kBlock / W;
-  __shared__ uint2 rec[kTile];
-  __shared__ uint32_t S[kBlock + 1];
-  const uint32_t t = threadIdx.x, f = t % W, grp = t / W;
-  double ll = 0.0, tot = 0.0;
-  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const uint2 d0 = tiles[tile], d1 = tiles[tile + 1];
-    const uint32_t s0 = d0.x, e0 = d0.y, nseg = d1.x - s0, count = d1.y - e0;
-    __syncthreads();
-    const bool staged = count <= kTile;
-    if (staged) {
-      for (uint32_t i = t; i < count; i += kBlock) rec[i] = csr[e0 + i];
-    }
-    for (uint32_t i = t; i <= nseg; i += kBlock) S[i] = rowptr[s0 + i] - e0;
-    __syncthreads();
-    for (uint32_t sg = grp; sg < nseg; sg += G) {
-      const uint32_t row = s0 + sg;
-      const float F = staged ? seg_dot_batch<W>(rec, S[sg], S[sg + 1], X, f)
-                             : seg_dot_batch<W>(csr + e0, S[sg], S[sg + 1], X, f);
-      const int32_t yi = Y[(size_t)row * W + f];
-      R[(size_t)row * W + f] = (F != 0.f) ? (float)yi / F : 0.f;
-      tot += (double)F;
-      if (F > 0.f) ll += (yi != 0 ? (double)yi * log((double)F) : 0.0) - (double)F;
-    }
-  }
-  batch_stats_finish<W>(ll, tot, partials, counter, results, ring);
+  FwdRatio<W> epi{Y, R};
+  spmm::walk<W>(rowptr, csr, tiles, ntiles, X, epi);
+  batch_stats_finish<W>(epi.ll, epi.tot, partials, counter, results, ring);
 }
 
-// Backprojection + update for one RHS block over the CSC column tiles; lanes
-// whose frame is finished (bit clear in *mask) keep their image.
+// Backprojection + update for one RHS block over the CSC column tiles.
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_bwd_batch(
     const uint32_t* __restrict__ colptr, const uint2* __restrict__ csc,
     const uint2* __restrict__ tiles, uint32_t ntiles, const float* __restrict__ R,
     const float* __restrict__ norm, float* __restrict__ X, const uint32_t* __restrict__ mask) {
-  constexpr int G = kBlock / W;
-  __shared__ uint2 rec[kTile];
-  __shared__ uint32_t S[kBlock + 1];
-  const uint32_t t = threadIdx.x, f = t % W, grp = t / W;
-  const bool live = (*mask >> f) & 1u;
-  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const uint2 d0 = tiles[tile], d1 = tiles[tile + 1];
-    const uint32_t s0 = d0.x, e0 = d0.y, nseg = d1.x - s0, count = d1.y - e0;
-    __syncthreads();
-    const bool staged = count <= kTile;
-    if (staged) {
-      for (uint32_t i = t; i < count; i += kBlock) rec[i] = csc[e0 + i];
-    }
-    for (uint32_t i = t; i <= nseg; i += kBlock) S[i] = colptr[s0 + i] - e0;
-    __syncthreads();
-    for (uint32_t sg = grp; sg < nseg; sg += G) {
-      const uint32_t col = s0 + sg;
-      const float u = staged ? seg_dot_batch<W>(rec, S[sg], S[sg + 1], R, f)
-                             : seg_dot_batch<W>(csc + e0, S[sg], S[sg + 1], R, f);
-      if (live) {
-        const float n = norm[col];
-        const size_t i = (size_t)col * W + f;
-        X[i] = X[i] * ((n != 0.f) ? u / n : u);
-      }
-    }
-  }
+  BwdUpdate<W> epi{norm, X, *mask};
+  spmm::walk<W>(colptr, csc, tiles, ntiles, R, epi);
 }
 
 // out[c*W + lane] = in[src(c)] for one frame: file-order image <-> block lane.
@@ -243,27 +200,6 @@ __global__ __launch_bounds__(kBlock) void k_lane_move_i32(const int32_t* __restr
                                                           int Wd, int ld, uint32_t n) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i < n) dst[(size_t)i * Wd + ld] = src[(size_t)i * Ws + ls];
-}
-
-std::vector<uint32_t> partition(const std::vector<uint32_t>& ptr, uint32_t T) {
-  const uint32_t n = (uint32_t)ptr.size() - 1;
-  std::vector<uint32_t> tiles{0};
-  uint32_t s = 0;
-  while (s < n) {
-    uint32_t e = s;
-    if (ptr[s + 1] - ptr[s] > T) {
-      e = s + 1;
-    } else {
-      uint64_t cnt = 0;
-      while (e < n && e - s < (uint32_t)kBlock && cnt + (ptr[e + 1] - ptr[e]) <= T) {
-        cnt += ptr[e + 1] - ptr[e];
-        ++e;
-      }
-    }
-    tiles.push_back(e);
-    s = e;
-  }
-  return tiles;
 }
 
 template <class T>
@@ -486,7 +422,8 @@ int mlem_batch_create(mlem_matrix* h, uint32_t frames, const int32_t* y, uint32_
     B->grid = (uint32_t)std::max(8, 8 * m.cus);
     // tiles (device-order pointers are on the host after open)
     prepare_tiles(m, m.stream);
-    const std::vector<uint32_t> tf = partition(m.h_rowptr, kTile), tb = partition(m.h_colptr, kTile);
+    const std::vector<uint32_t> tf = spmm::partition(m.h_rowptr, kTile),
+                                tb = spmm::partition(m.h_colptr, kTile);
     std::vector<uint2> df(tf.size()), db(tb.size());
     for (size_t i = 0; i < tf.size(); ++i) df[i] = make_uint2(tf[i], m.h_rowptr[tf[i]]);
     for (size_t i = 0; i < tb.size(); ++i) db[i] = make_uint2(tb[i], m.h_colptr[tb[i]]);

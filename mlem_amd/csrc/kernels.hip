@@ -478,11 +478,16 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
       val[j] = __builtin_nontemporal_load(dv2 + t + j * kBlock);
       li[j] = __builtin_nontemporal_load(dl2 + t + j * kBlock);
     }
+    // Only the rounds that hold dictionary entries gather (block-uniform
+    // count); the last round's spare lanes repeat the last entry.
+    const uint32_t rounds = __builtin_amdgcn_readfirstlane((nd + kBlock - 1) / kBlock);
     float g[EPT];
 #pragma unroll
-    for (int j = 0; j < EPT; ++j) g[j] = v[dict[g0 + min(t + j * kBlock, nd - 1)]];
+    for (int j = 0; j < EPT; ++j)
+      if (j < (int)rounds) g[j] = v[dict[g0 + min(t + j * kBlock, nd - 1)]];
 #pragma unroll
-    for (int j = 0; j < EPT; ++j) V[t + j * kBlock] = g[j];
+    for (int j = 0; j < EPT; ++j)
+      if (j < (int)rounds) V[t + j * kBlock] = g[j];
   }
   if (t < nseg) S[t + 1] = send;
   if (t == 0) S[0] = 0;

@@ -49,6 +49,10 @@ def lib() -> C.CDLL:
             "orc_f64_mlem_scaled": (C.c_int, [_u32, _u32, _P, _P, _P, _P, _P, C.c_int, C.c_int,
                                               _P, _P]),
             "orc_quantize": (None, [_u32, _P, _P, C.c_int, _P, _P]),
+            "orc_f64_mlem_par": (C.c_int, [_u32, _u32, _P, _P, _P, _P, _P, C.c_int, C.c_int,
+                                           C.c_int, _P, _P]),
+            "orc_f64_unroll": (C.c_int, [_u32, _u32, _P, _P, _P, _P, _P, C.c_int, _P, _P, _P,
+                                         _P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -209,3 +213,38 @@ def f64_mlem_scaled(rowidx, elements, rscale, cols, y, iters, x0=None):
     if rc:
         raise MemoryError("oracle allocation failed")
     return x, ll[:iters], ft[:iters]
+
+
+def f64_mlem_par(rowidx, elements, cols, y, iters, rscale=None, x0=None, threads=0):
+    """FP64 MLEM parallel over rows (large matrices); rscale as in
+    f64_mlem_scaled (None: unscaled). Same results as f64_mlem up to the
+    order of FP64 sums."""
+    rowidx, el = _mat(rowidx, elements)
+    rs = None if rscale is None else np.ascontiguousarray(rscale, np.float32)
+    y = np.ascontiguousarray(y, np.float64)
+    x = np.zeros(cols, np.float64) if x0 is None else np.array(x0, np.float64, copy=True)
+    n = max(iters, 1)
+    ll, ft = np.zeros(n), np.zeros(n)
+    rc = lib().orc_f64_mlem_par(rowidx.size, cols, _p(rowidx), _p(el), _p(rs), _p(y), _p(x),
+                                iters, 0 if x0 is None else 1, threads, _p(ll), _p(ft))
+    if rc:
+        raise MemoryError("oracle allocation failed")
+    return x, ll[:iters], ft[:iters]
+
+
+def f64_unroll(rowidx, elements, cols, y, x0, K, gK=None, rscale=None):
+    """K unrolled MLEM iterations in FP64 from x0 with real counts y.
+    Returns x_K, or (x_K, dL/dy, dL/dx0) when gK = dL/dx_K is given."""
+    rowidx, el = _mat(rowidx, elements)
+    rs = None if rscale is None else np.ascontiguousarray(rscale, np.float32)
+    y = np.ascontiguousarray(y, np.float64)
+    x0 = np.ascontiguousarray(x0, np.float64)
+    xK = np.empty(cols, np.float64)
+    g = None if gK is None else np.ascontiguousarray(gK, np.float64)
+    gy = None if gK is None else np.empty(rowidx.size, np.float64)
+    gx0 = None if gK is None else np.empty(cols, np.float64)
+    rc = lib().orc_f64_unroll(rowidx.size, cols, _p(rowidx), _p(el), _p(rs), _p(y), _p(x0), K,
+                              _p(xK), _p(g), _p(gy), _p(gx0))
+    if rc:
+        raise MemoryError("oracle allocation failed")
+    return xK if gK is None else (xK, gy, gx0)

@@ -150,3 +150,20 @@ def test_quantization_error_bound(kind):
     assert np.all(np.abs(deq - a) <= tol)
     # and the bound is not vacuous: some element uses more than half of it
     assert np.max(np.abs(deq - a) / np.maximum(tol, 1e-30)) > 0.5
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_parallel_f64_matches_sequential(threads):
+    rng = np.random.default_rng(7)
+    ends, el, cols = _matrix(rng, rows=300, cols=200, with_special=False)
+    y = rng.integers(0, 30, ends.size).astype(np.float64)
+    s = rng.uniform(0.5, 2, ends.size).astype(np.float32)
+    for rs in (None, s):
+        if rs is None:
+            xa, la, fa = orc.f64_mlem(ends, el, cols, y, 9)
+        else:
+            xa, la, fa = orc.f64_mlem_scaled(ends, el, rs, cols, y, 9)
+        xb, lb, fb = orc.f64_mlem_par(ends, el, cols, y, 9, rscale=rs, threads=threads)
+        assert np.max(np.abs(xa - xb) / np.maximum(np.abs(xa), 1e-300)) < 1e-12
+        assert np.max(np.abs(la - lb) / np.abs(la)) < 1e-13
+        assert np.max(np.abs(fa - fb) / np.abs(fa)) < 1e-13
