@@ -42,6 +42,35 @@ def kernel_bytes(rows: int, cols: int, nnz: int, precision: str = "fp32") -> dic
     return {"fwd": fwd, "bwd": bwd, "iteration": fwd + bwd}
 
 
+def unrolled_run(A, prob, frames: int, K: int, checkpoint: int) -> dict:
+    """Differentiable unrolled MLEM on the same matrix: `frames` sinograms
+    with relaxed (real) counts, K iterations from x0 = sum(y)/sum(n), loss
+    0.5 ||x_K - x_true||^2; times the forward pass and the reverse pass
+    (dL/dy, dL/dx0 with recomputation from checkpoints)."""
+    import mlem_amd as M
+    rng = np.random.default_rng(4000)
+    Y = np.stack([rng.poisson(prob.ybar * s) + rng.uniform(0, 0.5, prob.ybar.size)
+                  for s in rng.uniform(0.5, 2.0, frames)]).astype(np.float32)
+    U = M.Unrolled(A, Y, None, K, checkpoint_interval=checkpoint)  # warm-up
+    U.backward(U.xK - prob.x_true)
+    U.close()
+    M.device_sync()
+    t0 = time.perf_counter()
+    U = M.Unrolled(A, Y, None, K, checkpoint_interval=checkpoint)
+    M.device_sync()
+    t1 = time.perf_counter()
+    gy, gx0 = U.backward(U.xK - prob.x_true)
+    M.device_sync()
+    t2 = time.perf_counter()
+    info = U.info()
+    U.close()
+    return {"frames": frames, "K": K, **info,
+            "forward_ms": 1e3 * (t1 - t0), "backward_ms": 1e3 * (t2 - t1),
+            "note": "host-to-device of y and device-to-host of x_K, dL/dy, dL/dx0 "
+                    "included in the two times",
+            "grad_finite": bool(np.all(np.isfinite(gy)) and np.all(np.isfinite(gx0)))}
+
+
 def fp64_check(A, prob, precision: str, iters: int) -> dict:
     """`iters` GPU iterations against FP64 MLEM on the original values (the
     accuracy of the stored precision) and, for 16-bit storage, on the
@@ -160,6 +189,9 @@ def main() -> int:
                          "hardware byte counters under rocprofv3 --pmc)")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16"],
                     help="matrix value storage (16-bit: per-row fp32 scale)")
+    ap.add_argument("--unrolled-frames", type=int, default=8,
+                    help="frames of the unrolled forward/backward timing (K = 20, "
+                         "checkpoints every 5 iterations); 0 = skip")
     ap.add_argument("--fp64-iters", type=int, default=0,
                     help="also compare this many iterations with FP64 MLEM (oracle, all "
                          "host cores); 0 = skip")
@@ -252,6 +284,14 @@ def main() -> int:
             batched = {"error": f"{type(e).__name__}: {e}"}
         print(f"[bench] batched: {batched}", file=sys.stderr, flush=True)
 
+    unrolled = None
+    if args.unrolled_frames > 0 and args.precision == "fp32" and args.mode != "atomic":
+        try:
+            unrolled = unrolled_run(A, prob, args.unrolled_frames, 20, 5)
+        except Exception as e:  # reported in the line, never hides the plain result
+            unrolled = {"error": f"{type(e).__name__}: {e}"}
+        print(f"[bench] unrolled: {unrolled}", file=sys.stderr, flush=True)
+
     accuracy = None
     if args.fp64_iters > 0:
         accuracy = fp64_check(A, prob, args.precision, args.fp64_iters)
@@ -294,6 +334,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "parity": parity,
         "batched": batched,
+        "unrolled": unrolled,
         "fp64_check": accuracy,
     }
     print(json.dumps(line), flush=True)

@@ -436,6 +436,71 @@ struct ValPair<kBF16> {
   __device__ static float hi(W w) { return __uint_as_float(w & 0xFFFF0000u); }
 };
 
+// Raw buffer resource over [base, base + bytes): 32-bit byte offsets held in
+// one VGPR (no 64-bit address per load), loads at or past `bytes` return 0.
+// Word 3 is the gfx9-family raw-buffer configuration.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                           0x00020000);
+}
+constexpr int kCacheNT = 2;  // cache-policy bit: non-temporal (streamed once)
+// Dictionary tiles: issue the element stream before the dictionary gathers.
+constexpr bool kStreamFirst = true;
+
+template <class W, int AUX>
+struct BufferLoad;
+template <int AUX>
+struct BufferLoad<uint32_t, AUX> {
+  __device__ static uint32_t load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(uint32_t, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, AUX));
+  }
+};
+template <int AUX>
+struct BufferLoad<u32x2, AUX> {
+  __device__ static u32x2 load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, AUX));
+  }
+};
+
+// Loads of one dictionary tile: R rounds of 256 dictionary indices (rounds
+// past the dictionary read 0, a valid index), the R gathers they address,
+// then the H element pairs (values and dictionary positions) that stay in
+// registers; the gathered values go to V last, so the element stream is in
+// flight while the gathers complete.
+template <int R, int H, class W>
+__device__ __forceinline__ void load_dict_tile(__amdgpu_buffer_rsrc_t rdict,
+                                               __amdgpu_buffer_rsrc_t rv,
+                                               __amdgpu_buffer_rsrc_t rval,
+                                               __amdgpu_buffer_rsrc_t rlix, uint32_t t, float* V,
+                                               W (&val)[H], uint32_t (&li)[H]) {
+  if constexpr (kStreamFirst) {
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      val[j] = BufferLoad<W, kCacheNT>::load(rval, (t + j * kBlock) * (uint32_t)sizeof(W));
+      li[j] = BufferLoad<uint32_t, kCacheNT>::load(rlix, (t + j * kBlock) * 4u);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  uint32_t ix[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) ix[j] = BufferLoad<uint32_t, 0>::load(rdict, (t + j * kBlock) * 4u);
+  __builtin_amdgcn_sched_barrier(0);  // one wait between the index and the gather loads
+  float g[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) g[j] = __uint_as_float(BufferLoad<uint32_t, 0>::load(rv, ix[j] * 4u));
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (!kStreamFirst) {
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      val[j] = BufferLoad<W, kCacheNT>::load(rval, (t + j * kBlock) * (uint32_t)sizeof(W));
+      li[j] = BufferLoad<uint32_t, kCacheNT>::load(rlix, (t + j * kBlock) * 4u);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+#pragma unroll
+  for (int j = 0; j < R; ++j) V[t + j * kBlock] = g[j];
+}
+
 template <int EPT, int VK, class Epi>
 __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
                                           const uint2* __restrict__ el,
@@ -443,15 +508,15 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
                                           const void* __restrict__ dval,
                                           const uint16_t* __restrict__ dlix,
                                           const uint32_t* __restrict__ dict,
-                                          const float* __restrict__ v, Epi& epi) {
+                                          const float* __restrict__ v, uint32_t nv, Epi& epi) {
   static_assert(EPT % 2 == 0, "elements are loaded in pairs");
   using VP = ValPair<VK>;
   using W = typename VP::W;
   constexpr uint32_t T = kBlock * EPT;
   constexpr int H = EPT / 2;
   constexpr int G = 16;
-  __shared__ float P[T];
-  __shared__ float V[T];
+  __shared__ float V[T];  // gathered dictionary values
+  __shared__ float P[T];  // element products, tile order
   __shared__ uint32_t S[kBlock + 1];
   __shared__ float Sum[kBlock];
   __shared__ float red[kBlock / kWave];
@@ -470,24 +535,23 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
   W val[H];
   uint32_t li[H];
   if (count > 0) {  // block-uniform
-    const W* dv2 = reinterpret_cast<const W*>(
-        static_cast<const char*>(dval) + (size_t)a0 * (VK == kF32 ? 4 : 2));
-    const uint32_t* dl2 = reinterpret_cast<const uint32_t*>(dlix + a0);
-#pragma unroll
-    for (int j = 0; j < H; ++j) {
-      val[j] = __builtin_nontemporal_load(dv2 + t + j * kBlock);
-      li[j] = __builtin_nontemporal_load(dl2 + t + j * kBlock);
-    }
-    // Only the rounds that hold dictionary entries gather (block-uniform
-    // count); the last round's spare lanes repeat the last entry.
+    constexpr uint32_t vb = VK == kF32 ? 4 : 2;
+    const auto rval = buffer_rsrc(static_cast<const char*>(dval) + (size_t)a0 * vb, T * vb);
+    const auto rlix = buffer_rsrc(dlix + a0, T * 2);
+    const auto rdict = buffer_rsrc(dict + g0, nd * 4);
+    const auto rv = buffer_rsrc(v, nv * 4);
+    // Gather only as many 256-entry rounds as the dictionary needs, rounded
+    // up to a multiple of 4 (block-uniform branch; each variant issues all
+    // its loads before the first wait).
     const uint32_t rounds = __builtin_amdgcn_readfirstlane((nd + kBlock - 1) / kBlock);
-    float g[EPT];
-#pragma unroll
-    for (int j = 0; j < EPT; ++j)
-      if (j < (int)rounds) g[j] = v[dict[g0 + min(t + j * kBlock, nd - 1)]];
-#pragma unroll
-    for (int j = 0; j < EPT; ++j)
-      if (j < (int)rounds) V[t + j * kBlock] = g[j];
+    if (EPT <= 4 || rounds <= 4)
+      load_dict_tile<(EPT < 4 ? EPT : 4)>(rdict, rv, rval, rlix, t, V, val, li);
+    else if (EPT <= 8 || rounds <= 8)
+      load_dict_tile<(EPT < 8 ? EPT : 8)>(rdict, rv, rval, rlix, t, V, val, li);
+    else if (EPT <= 12 || rounds <= 12)
+      load_dict_tile<(EPT < 12 ? EPT : 12)>(rdict, rv, rval, rlix, t, V, val, li);
+    else
+      load_dict_tile<EPT>(rdict, rv, rval, rlix, t, V, val, li);
   }
   if (t < nseg) S[t + 1] = send;
   if (t == 0) S[0] = 0;
@@ -518,13 +582,13 @@ __global__ __launch_bounds__(kBlock) void k_fwd_dict(
     const uint32_t* __restrict__ rowptr, const uint2* __restrict__ csr,
     const uint4* __restrict__ tiles, uint32_t ntiles, const void* __restrict__ dval,
     const uint16_t* __restrict__ dlix, const uint32_t* __restrict__ dict,
-    const float* __restrict__ x, const int32_t* __restrict__ y, float* __restrict__ out,
-    const float* __restrict__ rscale, double2* __restrict__ partials,
+    const float* __restrict__ x, uint32_t cols, const int32_t* __restrict__ y,
+    float* __restrict__ out, const float* __restrict__ rscale, double2* __restrict__ partials,
     unsigned* __restrict__ counter, double2* __restrict__ results, unsigned ring) {
   const uint32_t tile = logical_block();
   FwdEpilogue<RATIO, STATS> epi{y, out, rscale};
   if (tile < ntiles)
-    dict_spmv<EPT, VK>(rowptr, csr, tiles, tile, dval, dlix, dict, x, epi);
+    dict_spmv<EPT, VK>(rowptr, csr, tiles, tile, dval, dlix, dict, x, cols, epi);
   if constexpr (STATS) stats_finish(epi.ll, epi.tot, partials, counter, results, ring);
 }
 
@@ -533,11 +597,12 @@ __global__ __launch_bounds__(kBlock) void k_bwd_dict(
     const uint32_t* __restrict__ colptr, const uint2* __restrict__ csc,
     const uint4* __restrict__ tiles, uint32_t ntiles, const void* __restrict__ dval,
     const uint16_t* __restrict__ dlix, const uint32_t* __restrict__ dict,
-    const float* __restrict__ r, const float* __restrict__ norm, float* __restrict__ out) {
+    const float* __restrict__ r, uint32_t rows, const float* __restrict__ norm,
+    float* __restrict__ out) {
   const uint32_t tile = logical_block();
   if (tile >= ntiles) return;
   BwdEpilogue<UPDATE> epi{norm, out};
-  dict_spmv<EPT, VK>(colptr, csc, tiles, tile, dval, dlix, dict, r, epi);
+  dict_spmv<EPT, VK>(colptr, csc, tiles, tile, dval, dlix, dict, r, rows, epi);
 }
 
 // Dictionary construction, one block per tile. Pass 1 (tiles4 == nullptr)
@@ -887,15 +952,15 @@ void fwd_dv(const DevMatrix& m, const float* x, const int32_t* y, float* out, bo
   if (stats && grid.x > st->max_blocks) raise(MLEM_ESTATE, "statistics buffer too small for launch");
   if (stats)
     hipLaunchKernelGGL((k_fwd_dict<EPT, VK, true, true>), grid, block, 0, s, m.rowptr, m.csr,
-                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, x, y, out, m.rscale,
+                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, x, m.cols, y, out, m.rscale,
                        st->partials, st->counter, st->results, st->ring);
   else if (ratio)
     hipLaunchKernelGGL((k_fwd_dict<EPT, VK, true, false>), grid, block, 0, s, m.rowptr, m.csr,
-                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, x, y, out, m.rscale, nullptr,
+                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, x, m.cols, y, out, m.rscale, nullptr,
                        nullptr, nullptr, 0u);
   else
     hipLaunchKernelGGL((k_fwd_dict<EPT, VK, false, false>), grid, block, 0, s, m.rowptr, m.csr,
-                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, x, y, out, m.rscale, nullptr,
+                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, x, m.cols, y, out, m.rscale, nullptr,
                        nullptr, nullptr, 0u);
 }
 
@@ -904,6 +969,7 @@ void fwd_d(const DevMatrix& m, const float* x, const int32_t* y, float* out, boo
            const StatsState* st, hipStream_t s) {
   const DictFormat& D = m.dfwd;
   if (D.ept != EPT || !D.tiles) raise(MLEM_ESTATE, "forward dictionary not prepared");
+  if (m.cols >= (1u << 30)) raise(MLEM_EINVAL, "dictionary kernels need cols < 2^30");
   switch (D.vk) {
     case kF32: fwd_dv<EPT, kF32>(m, x, y, out, ratio, st, s); break;
     case kF16: fwd_dv<EPT, kF16>(m, x, y, out, ratio, st, s); break;
@@ -918,10 +984,10 @@ void bwd_dv(const DevMatrix& m, const float* r, const float* norm, float* out, b
   const dim3 grid(tile_grid(D.ntiles)), block(kBlock);
   if (update)
     hipLaunchKernelGGL((k_bwd_dict<EPT, VK, true>), grid, block, 0, s, m.colptr, m.csc, D.tiles,
-                       D.ntiles, D.dval, D.dlix, D.dict, r, norm, out);
+                       D.ntiles, D.dval, D.dlix, D.dict, r, m.rows, norm, out);
   else
     hipLaunchKernelGGL((k_bwd_dict<EPT, VK, false>), grid, block, 0, s, m.colptr, m.csc, D.tiles,
-                       D.ntiles, D.dval, D.dlix, D.dict, r, norm, out);
+                       D.ntiles, D.dval, D.dlix, D.dict, r, m.rows, norm, out);
 }
 
 template <int EPT>
@@ -929,6 +995,7 @@ void bwd_d(const DevMatrix& m, const float* r, const float* norm, float* out, bo
            hipStream_t s) {
   const DictFormat& D = m.dbwd;
   if (D.ept != EPT || !D.tiles) raise(MLEM_ESTATE, "backward dictionary not prepared");
+  if (m.rows >= (1u << 30)) raise(MLEM_EINVAL, "dictionary kernels need rows < 2^30");
   switch (D.vk) {
     case kF32: bwd_dv<EPT, kF32>(m, r, norm, out, update, s); break;
     case kF16: bwd_dv<EPT, kF16>(m, r, norm, out, update, s); break;
