@@ -134,9 +134,10 @@ struct BwdUpdate {
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_fwd_batch(
     const uint32_t* __restrict__ rowptr, const uint2* __restrict__ csr,
-    const uint2* __restrict__ tiles, uint32_t ntiles, const float* __restrict__ X,
-    const int32_t* __restrict__ Y, float* __restrict__ R, double2* __restrict__ partials,
-    unsigned* __restrict__ counter, double2* __restrict__ results, unsigned ring) {
+    const uint2* __restrict__ tiles, uint32_t ntiles,
+    const float* __restrict__ X, const int32_t* __restrict__ Y, float* __restrict__ R,
+    double2* __restrict__ partials, unsigned* __restrict__ counter,
+    double2* __restrict__ results, unsigned ring) {
   FwdRatio<W> epi{Y, R};
   spmm::walk<W>(rowptr, csr, tiles, ntiles, X, epi);
   batch_stats_finish<W>(epi.ll, epi.tot, partials, counter, results, ring);
@@ -146,8 +147,9 @@ __global__ __launch_bounds__(kBlock) void k_fwd_batch(
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_bwd_batch(
     const uint32_t* __restrict__ colptr, const uint2* __restrict__ csc,
-    const uint2* __restrict__ tiles, uint32_t ntiles, const float* __restrict__ R,
-    const float* __restrict__ norm, float* __restrict__ X, const uint32_t* __restrict__ mask) {
+    const uint2* __restrict__ tiles, uint32_t ntiles,
+    const float* __restrict__ R, const float* __restrict__ norm, float* __restrict__ X,
+    const uint32_t* __restrict__ mask) {
   BwdUpdate<W> epi{norm, X, *mask};
   spmm::walk<W>(colptr, csc, tiles, ntiles, R, epi);
 }
@@ -301,7 +303,8 @@ template <int W>
 void launch_block(mlem_batch* B, RhsBlock& b, hipStream_t s) {
   const DevMatrix& m = B->mat->m;
   hipLaunchKernelGGL((k_fwd_batch<W>), dim3(B->grid), dim3(kBlock), 0, s, m.rowptr, m.csr,
-                     B->tiles_f, B->nt_f, b.X, b.Y, b.R, b.partials, b.counter, b.results, kRing);
+                     B->tiles_f, B->nt_f, b.X, b.Y, b.R, b.partials, b.counter, b.results,
+                     kRing);
   hipLaunchKernelGGL((k_bwd_batch<W>), dim3(B->grid), dim3(kBlock), 0, s, m.colptr, m.csc,
                      B->tiles_b, B->nt_b, b.R, m.norm, b.X, b.mask);
 }
