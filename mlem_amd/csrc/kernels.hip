@@ -446,6 +446,10 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, 
 constexpr int kCacheNT = 2;  // cache-policy bit: non-temporal (streamed once)
 // Dictionary tiles: issue the element stream before the dictionary gathers.
 constexpr bool kStreamFirst = true;
+#ifndef MLEM_DICT_ALIAS_MIN_EPT
+#define MLEM_DICT_ALIAS_MIN_EPT 32
+#endif
+constexpr int kDictAliasMinEPT = MLEM_DICT_ALIAS_MIN_EPT;
 
 template <class W, int AUX>
 struct BufferLoad;
@@ -501,6 +505,24 @@ __device__ __forceinline__ void load_dict_tile(__amdgpu_buffer_rsrc_t rdict,
   for (int j = 0; j < R; ++j) V[t + j * kBlock] = g[j];
 }
 
+// load_dict_tile with the smallest multiple of 4 rounds >= `rounds` (at most
+// EPT): a chain of block-uniform branches over fully unrolled variants.
+template <int R, int EPT, int H, class W>
+__device__ __forceinline__ void load_dict_rounds(uint32_t rounds, __amdgpu_buffer_rsrc_t rdict,
+                                                 __amdgpu_buffer_rsrc_t rv,
+                                                 __amdgpu_buffer_rsrc_t rval,
+                                                 __amdgpu_buffer_rsrc_t rlix, uint32_t t, float* V,
+                                                 W (&val)[H], uint32_t (&li)[H]) {
+  if constexpr (R >= EPT) {
+    load_dict_tile<EPT>(rdict, rv, rval, rlix, t, V, val, li);
+  } else {
+    if (rounds <= (uint32_t)R)
+      load_dict_tile<R>(rdict, rv, rval, rlix, t, V, val, li);
+    else
+      load_dict_rounds<R + 4, EPT>(rounds, rdict, rv, rval, rlix, t, V, val, li);
+  }
+}
+
 template <int EPT, int VK, class Epi>
 __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
                                           const uint2* __restrict__ el,
@@ -515,8 +537,12 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
   constexpr uint32_t T = kBlock * EPT;
   constexpr int H = EPT / 2;
   constexpr int G = 16;
-  __shared__ float V[T];  // gathered dictionary values
-  __shared__ float P[T];  // element products, tile order
+  // Large tiles store the products over the dictionary values (half the LDS,
+  // one more barrier); smaller tiles keep two buffers.
+  constexpr bool kAlias = EPT >= kDictAliasMinEPT;
+  __shared__ float V[T];                  // gathered dictionary values
+  __shared__ float Pbuf[kAlias ? 1 : T];  // element products, tile order
+  float* P = kAlias ? V : Pbuf;
   __shared__ uint32_t S[kBlock + 1];
   __shared__ float Sum[kBlock];
   __shared__ float red[kBlock / kWave];
@@ -544,19 +570,30 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
     // up to a multiple of 4 (block-uniform branch; each variant issues all
     // its loads before the first wait).
     const uint32_t rounds = __builtin_amdgcn_readfirstlane((nd + kBlock - 1) / kBlock);
-    if (EPT <= 4 || rounds <= 4)
-      load_dict_tile<(EPT < 4 ? EPT : 4)>(rdict, rv, rval, rlix, t, V, val, li);
-    else if (EPT <= 8 || rounds <= 8)
-      load_dict_tile<(EPT < 8 ? EPT : 8)>(rdict, rv, rval, rlix, t, V, val, li);
-    else if (EPT <= 12 || rounds <= 12)
-      load_dict_tile<(EPT < 12 ? EPT : 12)>(rdict, rv, rval, rlix, t, V, val, li);
-    else
-      load_dict_tile<EPT>(rdict, rv, rval, rlix, t, V, val, li);
+    load_dict_rounds<4, EPT>(rounds, rdict, rv, rval, rlix, t, V, val, li);
   }
   if (t < nseg) S[t + 1] = send;
   if (t == 0) S[0] = 0;
   __syncthreads();
-  if (count > 0) {
+  if constexpr (kAlias) {
+    float pr[EPT];
+    if (count > 0) {
+#pragma unroll
+      for (int j = 0; j < H; ++j) {
+        pr[2 * j] = VP::lo(val[j]) * V[li[j] & 0xFFFFu];
+        pr[2 * j + 1] = VP::hi(val[j]) * V[li[j] >> 16];
+      }
+    }
+    __syncthreads();  // every read of V is done before products overwrite it
+    if (count > 0) {
+#pragma unroll
+      for (int j = 0; j < H; ++j) {
+        const uint32_t q = 2 * (t + j * kBlock);
+        P[q] = pr[2 * j];
+        P[q + 1] = pr[2 * j + 1];
+      }
+    }
+  } else if (count > 0) {
 #pragma unroll
     for (int j = 0; j < H; ++j) {
       const uint32_t q = 2 * (t + j * kBlock);
@@ -1110,6 +1147,7 @@ void build_dict_vk(const std::vector<uint32_t>& ptr, const uint2* el, int ept, D
   switch (ept) {
     case 8: build_dict<8, VK>(ptr, el, D, s); break;
     case 16: build_dict<16, VK>(ptr, el, D, s); break;
+    case 32: build_dict<32, VK>(ptr, el, D, s); break;
     default: raise(MLEM_EINVAL, "unsupported dictionary tile size");
   }
 }
@@ -1171,6 +1209,7 @@ void launch_fwd(const DevMatrix& m, int mode, const float* x, const int32_t* y, 
     switch (m.fwd.ept) {
       case 8: fwd_d<8>(m, x, y, out, ratio, stats, s); break;
       case 16: fwd_d<16>(m, x, y, out, ratio, stats, s); break;
+      case 32: fwd_d<32>(m, x, y, out, ratio, stats, s); break;
       default: raise(MLEM_EINVAL, "unsupported forward dictionary tile size");
     }
   } else if (m.fwd.kind == kTiles) {
@@ -1202,6 +1241,7 @@ void launch_bwd(const DevMatrix& m, int mode, const float* r, const float* norm,
     switch (m.bwd.ept) {
       case 8: bwd_d<8>(m, r, norm, out, update, s); break;
       case 16: bwd_d<16>(m, r, norm, out, update, s); break;
+      case 32: bwd_d<32>(m, r, norm, out, update, s); break;
       default: raise(MLEM_EINVAL, "unsupported backward dictionary tile size");
     }
   } else if (m.bwd.kind == kTiles) {
