@@ -227,8 +227,8 @@ GroupLaunch make_launch(int kind, int param) {
       raise(MLEM_EINVAL, "tile elements per lane must be 4, 8 or 16");
     g.ept = param;
   } else if (kind == kDict) {
-    if (param != 8 && param != 16 && param != 32)
-      raise(MLEM_EINVAL, "dictionary tile elements per lane must be 8, 16 or 32");
+    if (param != 8 && param != 16 && param != 24 && param != 32 && param != 64)
+      raise(MLEM_EINVAL, "dictionary tile elements per lane must be 8, 16, 24, 32 or 64");
     g.ept = param;
   } else if (kind == kGroups) {
     if (param != 4 && param != 8 && param != 16 && param != 32 && param != 64)
@@ -243,7 +243,7 @@ GroupLaunch make_launch(int kind, int param) {
 // Candidate FAST-mode configurations of one projection.
 std::vector<GroupLaunch> launch_candidates() {
   std::vector<GroupLaunch> c;
-  for (int e : {8, 16, 32}) c.push_back(make_launch(kDict, e));
+  for (int e : {8, 16, 24, 32, 64}) c.push_back(make_launch(kDict, e));
   for (int e : {4, 8, 16}) c.push_back(make_launch(kTiles, e));
   for (int g : {8, 16, 32, 64}) c.push_back(make_launch(kGroups, g));
   return c;
@@ -367,8 +367,8 @@ void finish_open(mlem_matrix* h, unsigned flags) {
   h->norm_seconds = seconds_since(t0);
 
   const double avgRow = m.rows ? (double)m.nnz / m.rows : 1.0;
-  m.fwd = make_launch(kDict, 32);
-  m.bwd = make_launch(kDict, 32);
+  m.fwd = make_launch(kDict, 64);
+  m.bwd = make_launch(kDict, 64);
   m.scatter = make_launch(kGroups, lanes_for(avgRow));
   prepare_tiles(m, m.stream);
   if (env_int("MLEM_AUTOTUNE", 0) > 0) autotune(m);

@@ -447,7 +447,7 @@ constexpr int kCacheNT = 2;  // cache-policy bit: non-temporal (streamed once)
 // Dictionary tiles: issue the element stream before the dictionary gathers.
 constexpr bool kStreamFirst = true;
 #ifndef MLEM_DICT_ALIAS_MIN_EPT
-#define MLEM_DICT_ALIAS_MIN_EPT 32
+#define MLEM_DICT_ALIAS_MIN_EPT 24
 #endif
 constexpr int kDictAliasMinEPT = MLEM_DICT_ALIAS_MIN_EPT;
 
@@ -642,6 +642,12 @@ __global__ __launch_bounds__(kBlock) void k_bwd_dict(
   dict_spmv<EPT, VK>(colptr, csc, tiles, tile, dval, dlix, dict, r, rows, epi);
 }
 
+constexpr uint32_t pow2_ceil(uint32_t n) {
+  uint32_t p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
 // Dictionary construction, one block per tile. Pass 1 (tiles4 == nullptr)
 // counts the distinct indices of every tile; pass 2 writes the dictionary
 // and the {value, position} element arrays. Keys (index << 16 | element
@@ -656,7 +662,8 @@ __global__ __launch_bounds__(kBlock) void k_dict_build(const uint2* __restrict__
                                                        void* __restrict__ dval,
                                                        uint16_t* __restrict__ dlix) {
   constexpr uint32_t T = kBlock * EPT;
-  __shared__ unsigned long long K[T];
+  constexpr uint32_t TP = pow2_ceil(T);  // bitonic sort length
+  __shared__ unsigned long long K[TP];
   __shared__ uint32_t cnt[kBlock + 1];
   const uint32_t tile = blockIdx.x, t = threadIdx.x;
   uint32_t e0, e1, a0 = 0, g0 = 0;
@@ -675,12 +682,12 @@ __global__ __launch_bounds__(kBlock) void k_dict_build(const uint2* __restrict__
     if (!tiles4 && t == 0) nd_out[tile] = 0;
     return;
   }
-  for (uint32_t i = t; i < T; i += kBlock)
+  for (uint32_t i = t; i < TP; i += kBlock)
     K[i] = i < count ? (((unsigned long long)el[e0 + i].x << 16) | i) : ~0ull;
   __syncthreads();
-  for (uint32_t k = 2; k <= T; k <<= 1) {
+  for (uint32_t k = 2; k <= TP; k <<= 1) {
     for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t i = t; i < T; i += kBlock) {
+      for (uint32_t i = t; i < TP; i += kBlock) {
         const uint32_t ixj = i ^ j;
         if (ixj > i) {
           const unsigned long long a = K[i], b = K[ixj];
@@ -1147,7 +1154,9 @@ void build_dict_vk(const std::vector<uint32_t>& ptr, const uint2* el, int ept, D
   switch (ept) {
     case 8: build_dict<8, VK>(ptr, el, D, s); break;
     case 16: build_dict<16, VK>(ptr, el, D, s); break;
+    case 24: build_dict<24, VK>(ptr, el, D, s); break;
     case 32: build_dict<32, VK>(ptr, el, D, s); break;
+    case 64: build_dict<64, VK>(ptr, el, D, s); break;
     default: raise(MLEM_EINVAL, "unsupported dictionary tile size");
   }
 }
@@ -1209,7 +1218,9 @@ void launch_fwd(const DevMatrix& m, int mode, const float* x, const int32_t* y, 
     switch (m.fwd.ept) {
       case 8: fwd_d<8>(m, x, y, out, ratio, stats, s); break;
       case 16: fwd_d<16>(m, x, y, out, ratio, stats, s); break;
+      case 24: fwd_d<24>(m, x, y, out, ratio, stats, s); break;
       case 32: fwd_d<32>(m, x, y, out, ratio, stats, s); break;
+      case 64: fwd_d<64>(m, x, y, out, ratio, stats, s); break;
       default: raise(MLEM_EINVAL, "unsupported forward dictionary tile size");
     }
   } else if (m.fwd.kind == kTiles) {
@@ -1241,7 +1252,9 @@ void launch_bwd(const DevMatrix& m, int mode, const float* r, const float* norm,
     switch (m.bwd.ept) {
       case 8: bwd_d<8>(m, r, norm, out, update, s); break;
       case 16: bwd_d<16>(m, r, norm, out, update, s); break;
+      case 24: bwd_d<24>(m, r, norm, out, update, s); break;
       case 32: bwd_d<32>(m, r, norm, out, update, s); break;
+      case 64: bwd_d<64>(m, r, norm, out, update, s); break;
       default: raise(MLEM_EINVAL, "unsupported backward dictionary tile size");
     }
   } else if (m.bwd.kind == kTiles) {

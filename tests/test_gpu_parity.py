@@ -188,7 +188,8 @@ def test_edge_cases_empty_rows_and_columns():
 
 
 FAMILIES = [("tiles", 4), ("tiles", 8), ("tiles", 16), ("dict", 8), ("dict", 16),
-            ("groups", 4), ("groups", 16), ("groups", 64)]
+            ("dict", 24), ("dict", 32), ("dict", 64), ("groups", 4), ("groups", 16),
+            ("groups", 64)]
 
 
 @pytest.fixture(scope="module")
