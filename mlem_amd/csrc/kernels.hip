@@ -279,7 +279,7 @@ __device__ __forceinline__ void tile_spmv(const uint32_t* __restrict__ segptr,
                                           const float* __restrict__ v, Epi& epi) {
   constexpr uint32_t T = kBlock * EPT;
   constexpr int G = 16;
-  __shared__ float P[T];
+  __shared__ __align__(16) float P[T];
   __shared__ uint32_t S[kBlock + 1];
   __shared__ float Sum[kBlock];
   __shared__ float red[kBlock / kWave];
@@ -334,8 +334,17 @@ __device__ __forceinline__ void tile_spmv(const uint32_t* __restrict__ segptr,
   const uint32_t lane = t % G, grp = t / G;
   for (uint32_t sg = grp; sg < nseg; sg += kBlock / G) {
     const uint32_t b = S[sg], end = S[sg + 1];
+    // aligned 16-byte reads of the segment's products; the first and last
+    // quads are masked to [b, end)
     float acc = 0.f;
-    for (uint32_t k = b + lane; k < end; k += G) acc += P[k];
+    for (uint32_t qd = (b >> 2) + lane; 4 * qd < end; qd += G) {
+      const float4 p = reinterpret_cast<const float4*>(P)[qd];
+      const uint32_t k = 4 * qd;
+      acc += (k >= b) ? p.x : 0.f;
+      acc += (k + 1 >= b && k + 1 < end) ? p.y : 0.f;
+      acc += (k + 2 >= b && k + 2 < end) ? p.z : 0.f;
+      acc += (k + 3 >= b && k + 3 < end) ? p.w : 0.f;
+    }
     acc = group_sum<G>(acc);
     if (lane == 0) Sum[sg] = acc;
   }
@@ -540,8 +549,8 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
   // Large tiles store the products over the dictionary values (half the LDS,
   // one more barrier); smaller tiles keep two buffers.
   constexpr bool kAlias = EPT >= kDictAliasMinEPT;
-  __shared__ float V[T];                  // gathered dictionary values
-  __shared__ float Pbuf[kAlias ? 1 : T];  // element products, tile order
+  __shared__ __align__(16) float V[T];                  // gathered dictionary values
+  __shared__ __align__(16) float Pbuf[kAlias ? 2 : T];  // element products, tile order
   float* P = kAlias ? V : Pbuf;
   __shared__ uint32_t S[kBlock + 1];
   __shared__ float Sum[kBlock];
@@ -587,26 +596,30 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
     __syncthreads();  // every read of V is done before products overwrite it
     if (count > 0) {
 #pragma unroll
-      for (int j = 0; j < H; ++j) {
-        const uint32_t q = 2 * (t + j * kBlock);
-        P[q] = pr[2 * j];
-        P[q + 1] = pr[2 * j + 1];
-      }
+      for (int j = 0; j < H; ++j)  // one 8-byte store per pair: no bank conflicts
+        reinterpret_cast<float2*>(P)[t + j * kBlock] = make_float2(pr[2 * j], pr[2 * j + 1]);
     }
   } else if (count > 0) {
 #pragma unroll
-    for (int j = 0; j < H; ++j) {
-      const uint32_t q = 2 * (t + j * kBlock);
-      P[q] = VP::lo(val[j]) * V[li[j] & 0xFFFFu];
-      P[q + 1] = VP::hi(val[j]) * V[li[j] >> 16];
-    }
+    for (int j = 0; j < H; ++j)
+      reinterpret_cast<float2*>(P)[t + j * kBlock] =
+          make_float2(VP::lo(val[j]) * V[li[j] & 0xFFFFu], VP::hi(val[j]) * V[li[j] >> 16]);
   }
   __syncthreads();
   const uint32_t lane = t % G, grp = t / G;
   for (uint32_t sg = grp; sg < nseg; sg += kBlock / G) {
     const uint32_t b = S[sg], end = S[sg + 1];
+    // aligned 16-byte reads of the segment's products; the first and last
+    // quads are masked to [b, end)
     float acc = 0.f;
-    for (uint32_t k = b + lane; k < end; k += G) acc += P[k];
+    for (uint32_t qd = (b >> 2) + lane; 4 * qd < end; qd += G) {
+      const float4 p = reinterpret_cast<const float4*>(P)[qd];
+      const uint32_t k = 4 * qd;
+      acc += (k >= b) ? p.x : 0.f;
+      acc += (k + 1 >= b && k + 1 < end) ? p.y : 0.f;
+      acc += (k + 2 >= b && k + 2 < end) ? p.z : 0.f;
+      acc += (k + 3 >= b && k + 3 < end) ? p.w : 0.f;
+    }
     acc = group_sum<G>(acc);
     if (lane == 0) Sum[sg] = acc;
   }
