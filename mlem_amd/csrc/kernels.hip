@@ -45,11 +45,28 @@ __device__ __forceinline__ uint32_t logical_block() {
   return (b & 7u) * (g >> 3) + (b >> 3);
 }
 
+// v from the lane that DPP control CTRL selects within the 16-lane row.
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over each group of G consecutive lanes; every lane of the group gets
+// the (fixed-order) sum. G = 16 uses row DPP moves (no LDS traffic): swap
+// within pairs, within quads, mirror the half-rows, mirror the row.
 template <int G>
 __device__ __forceinline__ float group_sum(float v) {
+  if constexpr (G == 16) {
+    v = v + dpp<0xB1>(v);   // quad_perm [1,0,3,2]
+    v = v + dpp<0x4E>(v);   // quad_perm [2,3,0,1]
+    v = v + dpp<0x141>(v);  // row_half_mirror
+    v = v + dpp<0x140>(v);  // row_mirror
+    return v;
+  } else {
 #pragma unroll
-  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
-  return v;
+    for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+    return v;
+  }
 }
 
 // Dot product of one sparse row (or CSC column) with a dense vector.
