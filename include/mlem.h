@@ -66,6 +66,12 @@ extern "C" {
  * MLEM only sees a_ij = s_i q_ij. */
 #define MLEM_OPEN_FP16 0x4u
 #define MLEM_OPEN_BF16 0x8u
+/* Files whose header sets symmetry flags (x 1, y 2, z 4; csr4matrix.hpp:
+ * 147-149) store a matrix reduced by the scanner's mirror symmetries. With
+ * this flag mlem_open regenerates the omitted elements on the host before the
+ * upload (see mlem_expand_symmetry_file); without it the stored elements are
+ * used as they are, as the reference's MLEM drivers do. */
+#define MLEM_OPEN_EXPAND_SYMMETRY 0x10u
 
 typedef struct mlem_matrix mlem_matrix; /* device-resident system matrix */
 typedef struct mlem_recon mlem_recon;   /* one reconstruction's state    */
@@ -96,6 +102,16 @@ MLEM_API int mlem_open(const char* path, unsigned flags, mlem_matrix** out);
  * symflags may be NULL. */
 MLEM_API int mlem_probe(const char* path, uint32_t* rows, uint32_t* cols,
                         uint64_t* nnz, uint32_t grid[3], uint32_t* symflags);
+
+/* Write the full matrix of a symmetry-reduced csr4 file (no GPU needed):
+ * every stored element plus its images under the symmetries its flags
+ * generate, as the reference's matrix viewer regenerates them
+ * (visualizematrix/symhandler.hpp:27-101, symhandler.cpp:33-115,
+ * dispcsr4spy.cpp:126-140): voxels mirrored in the grid, detectors mapped
+ * through the scanner's mirror tables, rows (detector pairs) re-sorted. One
+ * value per (row, column), the stored one first; columns ascending; the
+ * output's symmetry flags are cleared. A file without flags is copied. */
+MLEM_API int mlem_expand_symmetry_file(const char* in_path, const char* out_path);
 
 /* Same as mlem_open for a matrix already in host memory: rowidx holds the
  * csr4 row index (nRows u64 exclusive END offsets, csr4matrix.cpp:141-147,

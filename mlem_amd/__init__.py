@@ -25,7 +25,8 @@ from typing import Optional
 import numpy as np
 
 from ._native import (INIT_GIVEN, INIT_REFERENCE, MODE_ATOMIC, MODE_EXACT, MODE_FAST, MODES,
-                      OPEN_BF16, OPEN_FP16, OPEN_KEEP_ORDER, OPEN_NO_CSC, MlemError, Options, check, lib)
+                      OPEN_BF16, OPEN_EXPAND_SYMMETRY, OPEN_FP16, OPEN_KEEP_ORDER, OPEN_NO_CSC,
+                      MlemError, Options, check, lib)
 
 __all__ = [
     "Csr4Matrix", "Recon", "MlemError", "ELEMENT_DTYPE", "probe", "calcColumnSums", "initImage",
@@ -75,6 +76,11 @@ def probe(path: str) -> dict:
             "symflags": sym.value}
 
 
+def expand_symmetry_file(src: str, dst: str) -> None:
+    """Write the full matrix of a symmetry-reduced csr4 file (no GPU needed)."""
+    check(lib().mlem_expand_symmetry_file(os.fsencode(src), os.fsencode(dst)))
+
+
 class Csr4Matrix:
     """Device-resident csr4 system matrix (Csr4Matrix, csr4matrix.hpp:30-40).
 
@@ -86,7 +92,8 @@ class Csr4Matrix:
     """
 
     def __init__(self, path: Optional[str] = None, *, no_csc: bool = False,
-                 keep_order: bool = False, precision: str = "fp32", _handle=None):
+                 keep_order: bool = False, precision: str = "fp32",
+                 expand_symmetry: bool = False, _handle=None):
         self._h = None
         self.precision = precision
         if _handle is not None:
@@ -95,8 +102,10 @@ class Csr4Matrix:
             if path is None:
                 raise ValueError("path required")
             h = C.c_void_p()
-            check(lib().mlem_open(os.fsencode(path), _flags(no_csc, keep_order, precision),
-                                  C.byref(h)))
+            flags = _flags(no_csc, keep_order, precision)
+            if expand_symmetry:
+                flags |= OPEN_EXPAND_SYMMETRY
+            check(lib().mlem_open(os.fsencode(path), flags, C.byref(h)))
             self._h = h
         r, c, n = C.c_uint32(), C.c_uint32(), C.c_uint64()
         check(lib().mlem_dims(self._h, C.byref(r), C.byref(c), C.byref(n)))

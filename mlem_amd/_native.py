@@ -29,6 +29,7 @@ OPEN_NO_CSC = 0x1
 OPEN_KEEP_ORDER = 0x2
 OPEN_FP16 = 0x4
 OPEN_BF16 = 0x8
+OPEN_EXPAND_SYMMETRY = 0x10
 INIT_REFERENCE = 0
 INIT_GIVEN = 1
 
@@ -71,6 +72,7 @@ SIGNATURES = {
     "mlem_last_error": (C.c_char_p, []),
     "mlem_version": (C.c_char_p, []),
     "mlem_probe": (C.c_int, [C.c_char_p, _PU32, _PU32, C.POINTER(C.c_uint64), _PU32, _PU32]),
+    "mlem_expand_symmetry_file": (C.c_int, [C.c_char_p, C.c_char_p]),
     "mlem_open": (C.c_int, [C.c_char_p, C.c_uint, C.POINTER(_P)]),
     "mlem_open_host": (C.c_int, [_u32, _u32, _u64, _P, _P, _P, C.c_uint, C.POINTER(_P)]),
     "mlem_close": (None, [_P]),

@@ -23,6 +23,7 @@
 #include "csr4.hpp"
 #include "internal.hpp"
 #include "mlem.h"
+#include "symmetry.hpp"
 
 using namespace mlem;
 
@@ -574,14 +575,8 @@ int mlem_open(const char* path, unsigned flags, mlem_matrix** out) {
       }
     } fdc{fd};
     auto t0 = std::chrono::steady_clock::now();
-    std::unique_ptr<mlem_matrix> h(new_matrix(hd.nRows, hd.nColumns, hd.nnz));
-    h->m.grid[0] = hd.scanner.grid[0];
-    h->m.grid[1] = hd.scanner.grid[1];
-    h->m.grid[2] = hd.scanner.grid[2];
-    h->m.symflags = hd.flags & 7u;
-    upload_rowidx(h->m, rowidx.data());
     const std::string fn(path);
-    stream_to_device(h->m.csr, 8ull * hd.nnz, h->m.stream, [&](void* dst, uint64_t off, uint64_t n) {
+    auto read_at = [&](void* dst, uint64_t off, uint64_t n) {
       char* p = static_cast<char*>(dst);
       uint64_t done = 0;
       while (done < n) {
@@ -589,7 +584,33 @@ int mlem_open(const char* path, unsigned flags, mlem_matrix** out) {
         if (got <= 0) raise(MLEM_EIO, "Cannot read matrix elements from " + fn);
         done += (uint64_t)got;
       }
-    });
+    };
+    std::unique_ptr<mlem_matrix> h;
+    if ((flags & MLEM_OPEN_EXPAND_SYMMETRY) && (hd.flags & 7u)) {
+      // regenerate the omitted elements on the host, then upload the full matrix
+      std::vector<Csr4Element> el(hd.nnz);
+      if (hd.nnz) read_at(el.data(), 0, 8ull * hd.nnz);
+      std::vector<uint64_t> xrow;
+      std::vector<Csr4Element> xel;
+      expand_symmetry(hd.scanner, hd.flags & 7u, hd.nRows, hd.nColumns, rowidx, el.data(), xrow,
+                      xel);
+      el.clear();
+      el.shrink_to_fit();
+      h.reset(new_matrix(hd.nRows, hd.nColumns, xel.size()));
+      upload_rowidx(h->m, xrow.data());
+      const char* src = reinterpret_cast<const char*>(xel.data());
+      stream_to_device(h->m.csr, 8ull * xel.size(), h->m.stream,
+                       [&](void* dst, uint64_t off, uint64_t n) { std::memcpy(dst, src + off, n); });
+      h->m.symflags = 0;  // the matrix in HBM is complete
+    } else {
+      h.reset(new_matrix(hd.nRows, hd.nColumns, hd.nnz));
+      upload_rowidx(h->m, rowidx.data());
+      stream_to_device(h->m.csr, 8ull * hd.nnz, h->m.stream, read_at);
+      h->m.symflags = hd.flags & 7u;
+    }
+    h->m.grid[0] = hd.scanner.grid[0];
+    h->m.grid[1] = hd.scanner.grid[1];
+    h->m.grid[2] = hd.scanner.grid[2];
     h->load_seconds = seconds_since(t0);
     finish_open(h.get(), flags);
     *out = h.release();
@@ -609,6 +630,41 @@ int mlem_probe(const char* path, uint32_t* rows, uint32_t* cols, uint64_t* nnz, 
     if (grid)
       for (int k = 0; k < 3; ++k) grid[k] = hd.scanner.grid[k];
     if (symflags) *symflags = hd.flags & 7u;
+  });
+}
+
+int mlem_expand_symmetry_file(const char* in_path, const char* out_path) {
+  return guarded([&] {
+    if (!in_path || !out_path) raise(MLEM_EINVAL, "null argument");
+    Csr4Header hd;
+    std::vector<uint64_t> rowidx;
+    int fd = -1;
+    csr4_read_header(in_path, hd, rowidx, &fd);
+    struct FdClose {
+      int fd;
+      ~FdClose() {
+        if (fd >= 0) close(fd);
+      }
+    } fdc{fd};
+    std::vector<Csr4Element> el(hd.nnz);
+    const uint64_t bytes = 8ull * hd.nnz;
+    char* p = reinterpret_cast<char*>(el.data());
+    for (uint64_t done = 0; done < bytes;) {
+      ssize_t got = pread(fd, p + done, bytes - done, (off_t)(hd.dataOffset + done));
+      if (got <= 0) raise(MLEM_EIO, std::string("Cannot read matrix elements from ") + in_path);
+      done += (uint64_t)got;
+    }
+    if (!(hd.flags & 7u)) {
+      csr4_write(out_path, hd.flags, hd.nRows, hd.nColumns, hd.nnz, hd.scanner, rowidx.data(),
+                 el.data());
+      return;
+    }
+    std::vector<uint64_t> xrow;
+    std::vector<Csr4Element> xel;
+    expand_symmetry(hd.scanner, hd.flags & 7u, hd.nRows, hd.nColumns, rowidx, el.data(), xrow,
+                    xel);
+    csr4_write(out_path, (uint8_t)(hd.flags & ~7u), hd.nRows, hd.nColumns, xel.size(),
+               hd.scanner, xrow.data(), xel.data());
   });
 }
 

@@ -9,7 +9,9 @@
 // a failed assert abort the process, this driver prints the message and
 // exits with status 1.
 //
-// Environment: MLEM_MODE = fast (default) | exact | atomic.
+// Environment: MLEM_MODE = fast (default) | exact | atomic;
+// MLEM_SYMMETRY = expand regenerates the elements that a csr4 file with
+// symmetry flags omits (default: use the stored elements, as the reference).
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
@@ -62,6 +64,12 @@ int main(int argc, char* argv[]) {
     else if (!std::strcmp(m, "atomic")) mode = MLEM_MODE_ATOMIC;
     else if (*m && std::strcmp(m, "fast")) return fail(std::string("unknown MLEM_MODE ") + m);
   }
+  unsigned open_flags = mode == MLEM_MODE_ATOMIC ? MLEM_OPEN_NO_CSC : 0u;
+  if (const char* s = std::getenv("MLEM_SYMMETRY")) {
+    if (!std::strcmp(s, "expand")) open_flags |= MLEM_OPEN_EXPAND_SYMMETRY;
+    else if (*s && std::strcmp(s, "stored"))
+      return fail(std::string("unknown MLEM_SYMMETRY ") + s);
+  }
 
   std::cout << "Matrix file: " << mtx << std::endl;
   std::cout << "Input file: " << in << std::endl;
@@ -77,7 +85,7 @@ int main(int argc, char* argv[]) {
   }
 
   mlem_matrix* m = nullptr;
-  if (mlem_open(mtx.c_str(), mode == MLEM_MODE_ATOMIC ? MLEM_OPEN_NO_CSC : 0u, &m) != MLEM_OK)
+  if (mlem_open(mtx.c_str(), open_flags, &m) != MLEM_OK)
     return fail(mlem_last_error());
   uint32_t rows = 0, cols = 0;
   uint64_t nnz = 0;
