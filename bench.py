@@ -113,6 +113,14 @@ def cpu_baseline(problem, iters: int, parity_x: np.ndarray | None):
            "sample": f"{iters} iterations of the fp32 sequential restatement of seqcsr4mlem "
                      f"(oracle/mlem_oracle.c) on the full benchmark matrix; median per-iteration "
                      f"time {med:.3f} s"}
+    # the OpenMP driver's restatement (openmpcsr4mlem.cpp:69-128) on the host
+    # cores this process may use (at most 16)
+    threads = min(16, len(os.sched_getaffinity(0)))
+    _, osecs = orc.omp_mlem(m.rowidx, m.elements, m.cols, problem.y, iters, threads=threads)
+    omed = float(np.median(osecs))
+    out["openmp"] = {"value": 1.0 / omed, "unit": "iterations/s", "cores": threads,
+                     "sample": f"{iters} iterations of the OpenMP restatement, median "
+                               f"{omed:.3f} s per iteration"}
     parity = None
     if parity_x is not None:
         d = np.abs(parity_x.astype(np.float64) - x.astype(np.float64))
@@ -150,7 +158,9 @@ def batched_run(A, prob, frames: int, width: int, plain_value: float) -> dict:
     wall = time.perf_counter() - t0
     fi = B.progress()["frame_iterations"] - done0
     its = [B.frame_info(f)[0] for f in range(frames)]
-    out = {"frames": frames, "block_width": width, "frame_iterations": int(fi),
+    out = {"frames": frames, "block_width": width or "autotuned",
+           "block_ms_by_width": B.tuning() if width == 0 else None,
+           "frame_iterations": int(fi),
            "batched_iterations": n + 1, "wall_s": wall,
            "frame_iterations_per_s": fi / wall,
            "speedup_vs_plain": (fi / wall) / plain_value,
@@ -183,7 +193,8 @@ def main() -> int:
     ap.add_argument("--launch", default="", help="family,param,family,param e.g. tiles,8,groups,32")
     ap.add_argument("--batched-frames", type=int, default=64,
                     help="frames of the batched (dynamic-frame) measurement; 0 = skip")
-    ap.add_argument("--batched-width", type=int, default=32)
+    ap.add_argument("--batched-width", type=int, default=0,
+                    help="widest RHS block (4..32); 0 = autotuned widths")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="first run the plain copy kernel over this many MiB (calibrates "
                          "hardware byte counters under rocprofv3 --pmc)")

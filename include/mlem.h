@@ -243,7 +243,10 @@ MLEM_API int mlem_recon_iterations(const mlem_recon* r);
 typedef struct mlem_batch mlem_batch;
 typedef struct mlem_batch_options {
   double rel_tol;   /* early-stop tolerance on the log-likelihood (0 = off) */
-  int block_width;  /* widest RHS block: 4, 8, 16 or 32 frames            */
+  int block_width;  /* widest RHS block: 4, 8, 16 or 32 frames; 0 =
+                       autotune: one block-iteration is timed at every width
+                       at creation and live frames are packed into the
+                       blocks that minimise blocks x measured time        */
   int use_graph;    /* 1: replay each iteration from a captured hipGraph  */
 } mlem_batch_options;
 MLEM_API void mlem_batch_options_default(mlem_batch_options* o);
@@ -261,6 +264,9 @@ MLEM_API int mlem_batch_frame_info(mlem_batch* b, uint32_t frame, int* iteration
 MLEM_API int mlem_batch_progress(const mlem_batch* b, uint32_t* active_frames,
                                  uint64_t* frame_iterations, int* block_width,
                                  uint32_t* blocks);
+/* Measured milliseconds of one block-iteration at widths 4, 8, 16, 32
+ * (zeros unless the batch was created with block_width 0). */
+MLEM_API int mlem_batch_tuning(const mlem_batch* b, double block_ms[4]);
 MLEM_API void mlem_batch_destroy(mlem_batch* b);
 
 /* ---- differentiable unrolled MLEM ----------------------------------------

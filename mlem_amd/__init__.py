@@ -367,6 +367,12 @@ class Batch:
         return {"active": a.value, "frame_iterations": fi.value, "block_width": w.value,
                 "blocks": nb.value}
 
+    def tuning(self) -> dict:
+        """Measured ms per block-iteration by width (block_width=0 batches)."""
+        ms = (C.c_double * 4)()
+        check(lib().mlem_batch_tuning(self._h, ms))
+        return {w: ms[i] for i, w in enumerate((4, 8, 16, 32))}
+
     def close(self) -> None:
         if getattr(self, "_h", None) is not None:
             lib().mlem_batch_destroy(self._h)

@@ -228,8 +228,8 @@ GroupLaunch make_launch(int kind, int param) {
       raise(MLEM_EINVAL, "tile elements per lane must be 4, 8 or 16");
     g.ept = param;
   } else if (kind == kDict) {
-    if (param != 8 && param != 16 && param != 24 && param != 32 && param != 64)
-      raise(MLEM_EINVAL, "dictionary tile elements per lane must be 8, 16, 24, 32 or 64");
+    if (param != 8 && param != 16 && param != 24 && param != 32 && param != 48 && param != 64)
+      raise(MLEM_EINVAL, "dictionary tile elements per lane must be 8, 16, 24, 32, 48 or 64");
     g.ept = param;
   } else if (kind == kGroups) {
     if (param != 4 && param != 8 && param != 16 && param != 32 && param != 64)
@@ -244,7 +244,7 @@ GroupLaunch make_launch(int kind, int param) {
 // Candidate FAST-mode configurations of one projection.
 std::vector<GroupLaunch> launch_candidates() {
   std::vector<GroupLaunch> c;
-  for (int e : {8, 16, 24, 32, 64}) c.push_back(make_launch(kDict, e));
+  for (int e : {8, 16, 24, 32, 48, 64}) c.push_back(make_launch(kDict, e));
   for (int e : {4, 8, 16}) c.push_back(make_launch(kTiles, e));
   for (int g : {8, 16, 32, 64}) c.push_back(make_launch(kGroups, g));
   return c;

@@ -238,6 +238,9 @@ struct mlem_batch {
   mlem_matrix* mat = nullptr;
   uint32_t F = 0;
   int wmax = 32;
+  // measured cost (ms) of one block-iteration at widths 4, 8, 16, 32
+  // (autotuned batches; 0 = not measured)
+  double wcost[4] = {0, 0, 0, 0};
   double tol = 0.0;
   bool use_graph = false;
   std::vector<int> budget, done;
@@ -322,14 +325,76 @@ void enqueue_iteration(mlem_batch* B, hipStream_t s) {
   check_launch("batched iteration launch");
 }
 
-// Place the still-active frames into the fewest blocks of the narrowest
-// width that holds them; finished frames' images move to final_x.
+// Measure one block-iteration (forward + backward) at every width on a
+// scratch block (images 1, counts 0, all lanes live); median of 3.
+void tune_widths(mlem_batch* B) {
+  const DevMatrix& m = B->mat->m;
+  hipEvent_t a, b;
+  MLEM_HIP(hipEventCreate(&a));
+  MLEM_HIP(hipEventCreate(&b));
+  struct Ev {
+    hipEvent_t a, b;
+    ~Ev() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } ev{a, b};
+  for (int i = 0, W = 4; i < 4; ++i, W *= 2) {
+    std::unique_ptr<RhsBlock> blk = new_block(B, W);
+    for (int l = 0; l < W; ++l)
+      hipLaunchKernelGGL(k_lane_fill, dim3(blocks_for(m.cols)), dim3(kBlock), 0, m.stream, blk->X,
+                         W, l, 1.f, m.cols);
+    const uint32_t all = W == 32 ? 0xFFFFFFFFu : ((1u << W) - 1u);
+    MLEM_HIP(hipMemcpyAsync(blk->mask, &all, 4, hipMemcpyHostToDevice, m.stream));
+    std::vector<float> t;
+    for (int r = 0; r < 4; ++r) {
+      MLEM_HIP(hipEventRecord(a, m.stream));
+      switch (W) {
+        case 4: launch_block<4>(B, *blk, m.stream); break;
+        case 8: launch_block<8>(B, *blk, m.stream); break;
+        case 16: launch_block<16>(B, *blk, m.stream); break;
+        default: launch_block<32>(B, *blk, m.stream); break;
+      }
+      MLEM_HIP(hipEventRecord(b, m.stream));
+      MLEM_HIP(hipEventSynchronize(b));
+      float ms = 0.f;
+      MLEM_HIP(hipEventElapsedTime(&ms, a, b));
+      if (r) t.push_back(ms);  // first launch warms up
+    }
+    check_launch("batch width tuning");
+    std::sort(t.begin(), t.end());
+    B->wcost[i] = t[t.size() / 2];
+  }
+}
+
+// Block width for A live frames: with measured costs, the width (<= wmax)
+// that minimises ceil(A/W) * cost(W); otherwise the narrowest power of two
+// that holds them (at most wmax).
+int choose_width(const mlem_batch* B, int A) {
+  if (B->wcost[0] > 0.0) {
+    int best = 4;
+    double bt = -1.0;
+    for (int i = 0, W = 4; i < 4 && W <= B->wmax; ++i, W *= 2) {
+      const double t = (double)((A + W - 1) / W) * B->wcost[i];
+      if (bt < 0.0 || t < bt) {
+        bt = t;
+        best = W;
+      }
+    }
+    return best;
+  }
+  int W = 4;
+  while (W < B->wmax && W < A) W *= 2;
+  return W;
+}
+
+// Place the still-active frames into the fewest blocks of the chosen width;
+// finished frames' images move to final_x.
 void repack(mlem_batch* B, bool force) {
   DevMatrix& m = B->mat->m;
   int A = 0;
   for (uint32_t f = 0; f < B->F; ++f) A += B->active[f] ? 1 : 0;
-  int W = 4;
-  while (W < B->wmax && W < A) W *= 2;
+  const int W = choose_width(B, A);
   const size_t nb = A == 0 ? 0 : (size_t)((A + W - 1) / W);
   bool same = !force && nb == B->blocks.size();
   for (auto& b : B->blocks) same = same && b->W == W;
@@ -403,12 +468,14 @@ int mlem_batch_create(mlem_matrix* h, uint32_t frames, const int32_t* y, uint32_
       o = *opts;
     else
       mlem_batch_options_default(&o);
-    if (o.block_width != 4 && o.block_width != 8 && o.block_width != 16 && o.block_width != 32)
-      raise(MLEM_EINVAL, "block_width must be 4, 8, 16 or 32");
+    if (o.block_width != 0 && o.block_width != 4 && o.block_width != 8 && o.block_width != 16 &&
+        o.block_width != 32)
+      raise(MLEM_EINVAL, "block_width must be 0 (autotune), 4, 8, 16 or 32");
+    const bool autotune = o.block_width == 0;
     std::unique_ptr<mlem_batch> B(new mlem_batch());
     B->mat = h;
     B->F = frames;
-    B->wmax = o.block_width;
+    B->wmax = autotune ? 32 : o.block_width;
     B->tol = o.rel_tol;
     B->use_graph = o.use_graph != 0;
     B->budget.assign(frames, 0x7FFFFFFF);
@@ -452,11 +519,11 @@ int mlem_batch_create(mlem_matrix* h, uint32_t frames, const int32_t* y, uint32_
       hipLaunchKernelGGL(k_lane_fill, dim3(blocks_for(m.cols)), dim3(kBlock), 0, m.stream,
                          B->final_x + (size_t)f * m.cols, 1, 0, x0[f], m.cols);
     }
+    if (autotune) tune_widths(B.get());
     // initial blocks: every frame with a budget
     int A = 0;
     for (uint32_t f = 0; f < frames; ++f) A += B->active[f];
-    int W = 4;
-    while (W < B->wmax && W < A) W *= 2;
+    const int W = choose_width(B.get(), A);
     const size_t nb = A == 0 ? 0 : (size_t)((A + W - 1) / W);
     size_t slot = 0;
     for (size_t i = 0; i < nb; ++i) B->blocks.push_back(new_block(B.get(), W));
@@ -572,6 +639,13 @@ int mlem_batch_progress(const mlem_batch* B, uint32_t* active_frames, uint64_t* 
     if (frame_iterations) *frame_iterations = B->frame_iterations;
     if (block_width) *block_width = B->blocks.empty() ? 0 : B->blocks[0]->W;
     if (blocks) *blocks = (uint32_t)B->blocks.size();
+  });
+}
+
+int mlem_batch_tuning(const mlem_batch* B, double block_ms[4]) {
+  return guarded([&] {
+    if (!B || !block_ms) raise(MLEM_EINVAL, "null argument");
+    for (int i = 0; i < 4; ++i) block_ms[i] = B->wcost[i];
   });
 }
 

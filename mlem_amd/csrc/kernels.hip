@@ -1186,6 +1186,7 @@ void build_dict_vk(const std::vector<uint32_t>& ptr, const uint2* el, int ept, D
     case 16: build_dict<16, VK>(ptr, el, D, s); break;
     case 24: build_dict<24, VK>(ptr, el, D, s); break;
     case 32: build_dict<32, VK>(ptr, el, D, s); break;
+    case 48: build_dict<48, VK>(ptr, el, D, s); break;
     case 64: build_dict<64, VK>(ptr, el, D, s); break;
     default: raise(MLEM_EINVAL, "unsupported dictionary tile size");
   }
@@ -1250,6 +1251,7 @@ void launch_fwd(const DevMatrix& m, int mode, const float* x, const int32_t* y, 
       case 16: fwd_d<16>(m, x, y, out, ratio, stats, s); break;
       case 24: fwd_d<24>(m, x, y, out, ratio, stats, s); break;
       case 32: fwd_d<32>(m, x, y, out, ratio, stats, s); break;
+      case 48: fwd_d<48>(m, x, y, out, ratio, stats, s); break;
       case 64: fwd_d<64>(m, x, y, out, ratio, stats, s); break;
       default: raise(MLEM_EINVAL, "unsupported forward dictionary tile size");
     }
@@ -1284,6 +1286,7 @@ void launch_bwd(const DevMatrix& m, int mode, const float* r, const float* norm,
       case 16: bwd_d<16>(m, r, norm, out, update, s); break;
       case 24: bwd_d<24>(m, r, norm, out, update, s); break;
       case 32: bwd_d<32>(m, r, norm, out, update, s); break;
+      case 48: bwd_d<48>(m, r, norm, out, update, s); break;
       case 64: bwd_d<64>(m, r, norm, out, update, s); break;
       default: raise(MLEM_EINVAL, "unsupported backward dictionary tile size");
     }

@@ -40,7 +40,7 @@ def tiny_batch(tiny_problem):
     A.close()
 
 
-@pytest.mark.parametrize("width", [4, 8, 32])
+@pytest.mark.parametrize("width", [4, 8, 32, 0])
 def test_batch_budgets_bitwise(tiny_batch, width):
     p, A, Y = tiny_batch
     m = p.matrix
@@ -127,3 +127,19 @@ def test_batch_argument_errors(tiny_batch):
         M.Batch(A, Y, block_width=12)
     with pytest.raises(M.MlemError):
         M.Batch(A, Y, np.full(Y.shape[0], -1, np.int32))
+
+
+def test_batch_autotuned_widths(tiny_batch):
+    p, A, Y = tiny_batch
+    B = M.Batch(A, Y, np.full(Y.shape[0], 3, np.int32), block_width=0)
+    ms = B.tuning()
+    assert set(ms) == {4, 8, 16, 32} and all(v > 0 for v in ms.values())
+    # the chosen width minimises blocks x measured time
+    w = B.progress()["block_width"]
+    cost = {k: -(-Y.shape[0] // k) * v for k, v in ms.items()}
+    assert cost[w] == min(cost.values())
+    B.run(10)
+    m = p.matrix
+    for f in range(Y.shape[0]):
+        assert np.array_equal(B.image(f), orc.mlem(m.rowidx, m.elements, m.cols, Y[f], 3)[0])
+    assert M.Batch(A, Y, block_width=32).tuning() == {4: 0.0, 8: 0.0, 16: 0.0, 32: 0.0}
