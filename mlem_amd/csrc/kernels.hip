@@ -74,10 +74,21 @@ template <int G, bool EXACT>
 __device__ __forceinline__ float seg_dot(const uint2* __restrict__ a, const float* __restrict__ v,
                                          uint32_t beg, uint32_t end, uint32_t lane) {
   if constexpr (EXACT) {
+    // one lane, ascending element order; the records and gathers of 8
+    // elements are issued together (clamped at the segment end), the products
+    // then added one by one
+    constexpr int U = 8;
     float acc = 0.f;
-    for (uint32_t k = beg; k < end; ++k) {
-      const u32x2 e = ld_stream(a + k);
-      acc = acc + __uint_as_float(e.y) * v[e.x];
+    for (uint32_t k = beg; k < end; k += U) {
+      u32x2 e[U];
+      float g[U];
+#pragma unroll
+      for (int j = 0; j < U; ++j) e[j] = ld_stream(a + min(k + j, end - 1));
+#pragma unroll
+      for (int j = 0; j < U; ++j) g[j] = v[e[j].x];
+#pragma unroll
+      for (int j = 0; j < U; ++j)
+        if (k + j < end) acc = acc + __uint_as_float(e[j].y) * g[j];
     }
     return acc;
   } else {
