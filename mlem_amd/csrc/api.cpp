@@ -368,8 +368,10 @@ void finish_open(mlem_matrix* h, unsigned flags) {
   h->norm_seconds = seconds_since(t0);
 
   const double avgRow = m.rows ? (double)m.nnz / m.rows : 1.0;
-  m.fwd = make_launch(kDict, 64);
-  m.bwd = make_launch(kDict, 64);
+  // 12288-element dictionary tiles: the largest that keep three blocks per CU
+  // (LDS-bound); mlem_autotune measures the alternatives on the real matrix.
+  m.fwd = make_launch(kDict, 48);
+  m.bwd = make_launch(kDict, 48);
   m.scatter = make_launch(kGroups, lanes_for(avgRow));
   prepare_tiles(m, m.stream);
   if (env_int("MLEM_AUTOTUNE", 0) > 0) autotune(m);

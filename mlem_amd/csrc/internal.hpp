@@ -62,11 +62,13 @@ struct DictFormat {
   int vk = 0;                 // value storage (ValueKind): fp32, fp16, bf16
   uint32_t ntiles = 0;
   uint4* tiles = nullptr;     // [ntiles+1] {first segment, first element,
-                              //  aligned element start, first dict entry}
-  uint32_t* dict = nullptr;   // distinct gather indices, sorted per tile
+                              //  aligned element start, first dict word}
+  uint32_t* dict = nullptr;   // distinct gather indices, sorted per tile, in
+                              //  64-entry slots (256 words per 256 entries)
+  uint32_t* dmeta = nullptr;  // per slot: first index | 0x80000000 if 32-bit
   void* dval = nullptr;       // element values, tile-aligned (4 or 2 bytes)
   uint16_t* dlix = nullptr;   // element positions in the tile dictionary
-  uint64_t ndict = 0, nval = 0;
+  uint64_t ndict = 0, nval = 0;  // dict words, element slots
 };
 
 // Storage precision of the matrix values.

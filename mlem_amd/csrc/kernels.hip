@@ -449,29 +449,54 @@ __device__ __forceinline__ float long_segment_sum(const uint2* __restrict__ el,
   return sum;
 }
 
-// Element values of a dictionary tile, loaded in pairs: two fp32 words, or
-// two 16-bit values (fp16 / bf16) in one word. The 16-bit forms widen
-// exactly to fp32.
-template <int VK>
-struct ValPair {
-  using W = u32x2;
-  __device__ static float lo(W w) { return __uint_as_float(w.x); }
-  __device__ static float hi(W w) { return __uint_as_float(w.y); }
+// Elements of a dictionary tile are loaded NP at a time (NP = 2 or 4): NP
+// fp32 words, or NP 16-bit values (fp16 / bf16) packed two to a word, and NP
+// 16-bit dictionary positions. The 16-bit value forms widen exactly to fp32.
+template <int NW>
+struct Words;
+template <>
+struct Words<1> {
+  using T = uint32_t;
+  __device__ static uint32_t at(T w, int) { return w; }
 };
 template <>
-struct ValPair<kF16> {
-  using W = uint32_t;
-  __device__ static float lo(W w) {
-    return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xFFFFu));
+struct Words<2> {
+  using T = u32x2;
+  __device__ static uint32_t at(T w, int i) { return w[i]; }
+};
+template <>
+struct Words<4> {
+  using T = u32x4;
+  __device__ static uint32_t at(T w, int i) { return w[i]; }
+};
+
+template <int VK, int NP>
+struct ValPack {
+  using WS = Words<VK == kF32 ? NP : NP / 2>;
+  using W = typename WS::T;
+  __device__ static float get(W w, int q) {
+    if constexpr (VK == kF32) {
+      return __uint_as_float(WS::at(w, q));
+    } else {
+      const uint32_t h = (WS::at(w, q >> 1) >> (16 * (q & 1))) & 0xFFFFu;
+      if constexpr (VK == kF16)
+        return (float)__builtin_bit_cast(_Float16, (uint16_t)h);
+      else
+        return __uint_as_float(h << 16);
+    }
   }
-  __device__ static float hi(W w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)); }
 };
-template <>
-struct ValPair<kBF16> {
-  using W = uint32_t;
-  __device__ static float lo(W w) { return __uint_as_float(w << 16); }
-  __device__ static float hi(W w) { return __uint_as_float(w & 0xFFFF0000u); }
+
+template <int NP>
+struct PosPack {
+  using WS = Words<NP / 2>;
+  using W = typename WS::T;
+  __device__ static uint32_t get(W w, int q) { return (WS::at(w, q >> 1) >> (16 * (q & 1))) & 0xFFFFu; }
 };
+
+// Elements per stream load: 16-byte value loads and 8-byte position loads
+// (fp32) measured 1-3% faster than pairs.
+constexpr int kDictPack = 4;
 
 // Raw buffer resource over [base, base + bytes): 32-bit byte offsets held in
 // one VGPR (no 64-bit address per load), loads at or past `bytes` return 0.
@@ -502,30 +527,53 @@ struct BufferLoad<u32x2, AUX> {
     return __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, AUX));
   }
 };
+template <int AUX>
+struct BufferLoad<u32x4, AUX> {
+  __device__ static u32x4 load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, AUX));
+  }
+};
 
-// Loads of one dictionary tile: R rounds of 256 dictionary indices (rounds
-// past the dictionary read 0, a valid index), the R gathers they address,
-// then the H element pairs (values and dictionary positions) that stay in
-// registers; the gathered values go to V last, so the element stream is in
-// flight while the gathers complete.
-template <int R, int H, class W>
-__device__ __forceinline__ void load_dict_tile(__amdgpu_buffer_rsrc_t rdict,
-                                               __amdgpu_buffer_rsrc_t rv,
+// The dictionary of one tile as stored: round j (256 entries) is four
+// 64-word slots, one per wave; slot s of the tile holds 64 32-bit indices, or,
+// when meta[s] has bit 31 clear, 64 16-bit offsets from meta[s] in its first
+// 32 words. Entries past the dictionary resolve to a valid index.
+struct DictSlots {
+  __amdgpu_buffer_rsrc_t words;  // the tile's slot words
+  const uint32_t* meta;          // the tile's slot descriptors
+  uint32_t wave, lane;           // wave index (uniform) and lane within it
+};
+
+// Loads of one dictionary tile: R rounds of 256 dictionary indices, the R
+// gathers they address, then the H element packs (values and dictionary
+// positions) that stay in registers; the gathered values go to V last, so
+// the element stream is in flight while the gathers complete.
+template <int R, int H, class W, class L>
+__device__ __forceinline__ void load_dict_tile(const DictSlots& ds, __amdgpu_buffer_rsrc_t rv,
                                                __amdgpu_buffer_rsrc_t rval,
                                                __amdgpu_buffer_rsrc_t rlix, uint32_t t, float* V,
-                                               W (&val)[H], uint32_t (&li)[H]) {
+                                               W (&val)[H], L (&li)[H]) {
   if constexpr (kStreamFirst) {
 #pragma unroll
     for (int j = 0; j < H; ++j) {
       val[j] = BufferLoad<W, kCacheNT>::load(rval, (t + j * kBlock) * (uint32_t)sizeof(W));
-      li[j] = BufferLoad<uint32_t, kCacheNT>::load(rlix, (t + j * kBlock) * 4u);
+      li[j] = BufferLoad<L, kCacheNT>::load(rlix, (t + j * kBlock) * (uint32_t)sizeof(L));
     }
     __builtin_amdgcn_sched_barrier(0);
   }
   uint32_t ix[R];
 #pragma unroll
-  for (int j = 0; j < R; ++j) ix[j] = BufferLoad<uint32_t, 0>::load(rdict, (t + j * kBlock) * 4u);
+  for (int j = 0; j < R; ++j) {
+    const uint32_t m = ds.meta[4 * j + ds.wave];
+    const uint32_t w = 256u * j + 64u * ds.wave + ((m >> 31) ? ds.lane : ds.lane >> 1);
+    ix[j] = BufferLoad<uint32_t, 0>::load(ds.words, w * 4u);
+  }
   __builtin_amdgcn_sched_barrier(0);  // one wait between the index and the gather loads
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const uint32_t m = ds.meta[4 * j + ds.wave];
+    if (!(m >> 31)) ix[j] = m + ((ix[j] >> (16 * (ds.lane & 1))) & 0xFFFFu);
+  }
   float g[R];
 #pragma unroll
   for (int j = 0; j < R; ++j) g[j] = __uint_as_float(BufferLoad<uint32_t, 0>::load(rv, ix[j] * 4u));
@@ -534,7 +582,7 @@ __device__ __forceinline__ void load_dict_tile(__amdgpu_buffer_rsrc_t rdict,
 #pragma unroll
     for (int j = 0; j < H; ++j) {
       val[j] = BufferLoad<W, kCacheNT>::load(rval, (t + j * kBlock) * (uint32_t)sizeof(W));
-      li[j] = BufferLoad<uint32_t, kCacheNT>::load(rlix, (t + j * kBlock) * 4u);
+      li[j] = BufferLoad<L, kCacheNT>::load(rlix, (t + j * kBlock) * (uint32_t)sizeof(L));
     }
     __builtin_amdgcn_sched_barrier(0);
   }
@@ -544,19 +592,19 @@ __device__ __forceinline__ void load_dict_tile(__amdgpu_buffer_rsrc_t rdict,
 
 // load_dict_tile with the smallest multiple of 4 rounds >= `rounds` (at most
 // EPT): a chain of block-uniform branches over fully unrolled variants.
-template <int R, int EPT, int H, class W>
-__device__ __forceinline__ void load_dict_rounds(uint32_t rounds, __amdgpu_buffer_rsrc_t rdict,
+template <int R, int EPT, int H, class W, class L>
+__device__ __forceinline__ void load_dict_rounds(uint32_t rounds, const DictSlots& ds,
                                                  __amdgpu_buffer_rsrc_t rv,
                                                  __amdgpu_buffer_rsrc_t rval,
                                                  __amdgpu_buffer_rsrc_t rlix, uint32_t t, float* V,
-                                                 W (&val)[H], uint32_t (&li)[H]) {
+                                                 W (&val)[H], L (&li)[H]) {
   if constexpr (R >= EPT) {
-    load_dict_tile<EPT>(rdict, rv, rval, rlix, t, V, val, li);
+    load_dict_tile<EPT>(ds, rv, rval, rlix, t, V, val, li);
   } else {
     if (rounds <= (uint32_t)R)
-      load_dict_tile<R>(rdict, rv, rval, rlix, t, V, val, li);
+      load_dict_tile<R>(ds, rv, rval, rlix, t, V, val, li);
     else
-      load_dict_rounds<R + 4, EPT>(rounds, rdict, rv, rval, rlix, t, V, val, li);
+      load_dict_rounds<R + 4, EPT>(rounds, ds, rv, rval, rlix, t, V, val, li);
   }
 }
 
@@ -567,12 +615,17 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
                                           const void* __restrict__ dval,
                                           const uint16_t* __restrict__ dlix,
                                           const uint32_t* __restrict__ dict,
+                                          const uint32_t* __restrict__ dmeta,
                                           const float* __restrict__ v, uint32_t nv, Epi& epi) {
-  static_assert(EPT % 2 == 0, "elements are loaded in pairs");
-  using VP = ValPair<VK>;
+  constexpr int NP = kDictPack;
+  static_assert(EPT % NP == 0, "elements are loaded in packs of NP");
+  using VP = ValPack<VK, NP>;
+  using LP = PosPack<NP>;
   using W = typename VP::W;
+  using L = typename LP::W;
+  using F = float __attribute__((ext_vector_type(NP)));
   constexpr uint32_t T = kBlock * EPT;
-  constexpr int H = EPT / 2;
+  constexpr int H = EPT / NP;
   constexpr int G = 16;
   // Large tiles store the products over the dictionary values (half the LDS,
   // one more barrier); smaller tiles keep two buffers.
@@ -586,7 +639,7 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
   const uint32_t t = threadIdx.x;
   const uint4 d0 = tiles[tile], d1 = tiles[tile + 1];
   const uint32_t s0 = d0.x, e0 = d0.y, a0 = d0.z, g0 = d0.w;
-  const uint32_t nseg = d1.x - s0, count = d1.y - e0, nd = d1.w - g0;
+  const uint32_t nseg = d1.x - s0, count = d1.y - e0, nwords = d1.w - g0;
   if (t < nseg) epi.prefetch(s0 + t);
   if (count > T) {
     const float sum = long_segment_sum<EPT>(el, v, e0, d1.y, red);
@@ -596,42 +649,49 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
   uint32_t send = 0;
   if (t < nseg) send = segptr[s0 + t + 1] - e0;
   W val[H];
-  uint32_t li[H];
+  L li[H];
   if (count > 0) {  // block-uniform
     constexpr uint32_t vb = VK == kF32 ? 4 : 2;
     const auto rval = buffer_rsrc(static_cast<const char*>(dval) + (size_t)a0 * vb, T * vb);
     const auto rlix = buffer_rsrc(dlix + a0, T * 2);
-    const auto rdict = buffer_rsrc(dict + g0, nd * 4);
+    // Rounds past the tile's own read words past its buffer (zeros) under
+    // the next tile's slot descriptors (the descriptor array is padded by
+    // 4*EPT zeros), which resolve to valid indices as well.
+    const DictSlots ds{buffer_rsrc(dict + g0, nwords * 4), dmeta + (g0 >> 6),
+                       (uint32_t)__builtin_amdgcn_readfirstlane(t / kWave), t % kWave};
     const auto rv = buffer_rsrc(v, nv * 4);
     // Gather only as many 256-entry rounds as the dictionary needs, rounded
     // up to a multiple of 4 (block-uniform branch; each variant issues all
     // its loads before the first wait).
-    const uint32_t rounds = __builtin_amdgcn_readfirstlane((nd + kBlock - 1) / kBlock);
-    load_dict_rounds<4, EPT>(rounds, rdict, rv, rval, rlix, t, V, val, li);
+    const uint32_t rounds = __builtin_amdgcn_readfirstlane(nwords / kBlock);
+    load_dict_rounds<4, EPT>(rounds, ds, rv, rval, rlix, t, V, val, li);
   }
   if (t < nseg) S[t + 1] = send;
   if (t == 0) S[0] = 0;
   __syncthreads();
+  // Products of pack j of this lane are elements NP*(t + j*kBlock) + q.
   if constexpr (kAlias) {
-    float pr[EPT];
+    F pr[H];
     if (count > 0) {
 #pragma unroll
-      for (int j = 0; j < H; ++j) {
-        pr[2 * j] = VP::lo(val[j]) * V[li[j] & 0xFFFFu];
-        pr[2 * j + 1] = VP::hi(val[j]) * V[li[j] >> 16];
-      }
+      for (int j = 0; j < H; ++j)
+#pragma unroll
+        for (int q = 0; q < NP; ++q) pr[j][q] = VP::get(val[j], q) * V[LP::get(li[j], q)];
     }
     __syncthreads();  // every read of V is done before products overwrite it
     if (count > 0) {
 #pragma unroll
-      for (int j = 0; j < H; ++j)  // one 8-byte store per pair: no bank conflicts
-        reinterpret_cast<float2*>(P)[t + j * kBlock] = make_float2(pr[2 * j], pr[2 * j + 1]);
+      for (int j = 0; j < H; ++j)  // one NP*4-byte store per pack: no bank conflicts
+        reinterpret_cast<F*>(P)[t + j * kBlock] = pr[j];
     }
   } else if (count > 0) {
 #pragma unroll
-    for (int j = 0; j < H; ++j)
-      reinterpret_cast<float2*>(P)[t + j * kBlock] =
-          make_float2(VP::lo(val[j]) * V[li[j] & 0xFFFFu], VP::hi(val[j]) * V[li[j] >> 16]);
+    for (int j = 0; j < H; ++j) {
+      F pr;
+#pragma unroll
+      for (int q = 0; q < NP; ++q) pr[q] = VP::get(val[j], q) * V[LP::get(li[j], q)];
+      reinterpret_cast<F*>(P)[t + j * kBlock] = pr;
+    }
   }
   __syncthreads();
   const uint32_t lane = t % G, grp = t / G;
@@ -660,13 +720,14 @@ __global__ __launch_bounds__(kBlock) void k_fwd_dict(
     const uint32_t* __restrict__ rowptr, const uint2* __restrict__ csr,
     const uint4* __restrict__ tiles, uint32_t ntiles, const void* __restrict__ dval,
     const uint16_t* __restrict__ dlix, const uint32_t* __restrict__ dict,
-    const float* __restrict__ x, uint32_t cols, const int32_t* __restrict__ y,
+    const uint32_t* __restrict__ dmeta, const float* __restrict__ x, uint32_t cols,
+    const int32_t* __restrict__ y,
     float* __restrict__ out, const float* __restrict__ rscale, double2* __restrict__ partials,
     unsigned* __restrict__ counter, double2* __restrict__ results, unsigned ring) {
   const uint32_t tile = logical_block();
   FwdEpilogue<RATIO, STATS> epi{y, out, rscale};
   if (tile < ntiles)
-    dict_spmv<EPT, VK>(rowptr, csr, tiles, tile, dval, dlix, dict, x, cols, epi);
+    dict_spmv<EPT, VK>(rowptr, csr, tiles, tile, dval, dlix, dict, dmeta, x, cols, epi);
   if constexpr (STATS) stats_finish(epi.ll, epi.tot, partials, counter, results, ring);
 }
 
@@ -675,12 +736,39 @@ __global__ __launch_bounds__(kBlock) void k_bwd_dict(
     const uint32_t* __restrict__ colptr, const uint2* __restrict__ csc,
     const uint4* __restrict__ tiles, uint32_t ntiles, const void* __restrict__ dval,
     const uint16_t* __restrict__ dlix, const uint32_t* __restrict__ dict,
-    const float* __restrict__ r, uint32_t rows, const float* __restrict__ norm,
+    const uint32_t* __restrict__ dmeta, const float* __restrict__ r, uint32_t rows,
+    const float* __restrict__ norm,
     float* __restrict__ out) {
   const uint32_t tile = logical_block();
   if (tile >= ntiles) return;
   BwdEpilogue<UPDATE> epi{norm, out};
-  dict_spmv<EPT, VK>(colptr, csc, tiles, tile, dval, dlix, dict, r, rows, epi);
+  dict_spmv<EPT, VK>(colptr, csc, tiles, tile, dval, dlix, dict, dmeta, r, rows, epi);
+}
+
+// Packs every 64-entry dictionary slot in place, one wave per slot: when the
+// slot's ascending indices span less than 2^16 they become 16-bit offsets
+// from the first (two per word, in the slot's first 32 words); meta[slot] =
+// first index, with bit 31 set when the slot keeps its 32-bit indices.
+// Unused entries at a tile's end hold 0 (below the first index) and become
+// offset 0.
+__global__ __launch_bounds__(kBlock) void k_dict_pack(uint32_t* __restrict__ dict,
+                                                      uint32_t* __restrict__ meta,
+                                                      uint64_t nslots) {
+  const uint64_t s = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  if (s >= nslots) return;  // whole waves
+  const uint32_t l = threadIdx.x % kWave;
+  const uint32_t v = dict[s * kWave + l];
+  const uint32_t base = __shfl(v, 0);
+  uint32_t hi = v;
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) hi = max(hi, (uint32_t)__shfl_xor(hi, off));
+  const bool wide = hi - base > 0xFFFFu;
+  if (!wide) {
+    const uint32_t o = v >= base ? v - base : 0u;
+    const uint32_t o1 = __shfl_down(o, 1);
+    if ((l & 1) == 0) dict[s * kWave + (l >> 1)] = o | (o1 << 16);
+  }
+  if (l == 0) meta[s] = base | (wide ? 0x80000000u : 0u);
 }
 
 constexpr uint32_t pow2_ceil(uint32_t n) {
@@ -1037,15 +1125,15 @@ void fwd_dv(const DevMatrix& m, const float* x, const int32_t* y, float* out, bo
   if (stats && grid.x > st->max_blocks) raise(MLEM_ESTATE, "statistics buffer too small for launch");
   if (stats)
     hipLaunchKernelGGL((k_fwd_dict<EPT, VK, true, true>), grid, block, 0, s, m.rowptr, m.csr,
-                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, x, m.cols, y, out, m.rscale,
+                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, x, m.cols, y, out, m.rscale,
                        st->partials, st->counter, st->results, st->ring);
   else if (ratio)
     hipLaunchKernelGGL((k_fwd_dict<EPT, VK, true, false>), grid, block, 0, s, m.rowptr, m.csr,
-                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, x, m.cols, y, out, m.rscale, nullptr,
+                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, x, m.cols, y, out, m.rscale, nullptr,
                        nullptr, nullptr, 0u);
   else
     hipLaunchKernelGGL((k_fwd_dict<EPT, VK, false, false>), grid, block, 0, s, m.rowptr, m.csr,
-                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, x, m.cols, y, out, m.rscale, nullptr,
+                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, x, m.cols, y, out, m.rscale, nullptr,
                        nullptr, nullptr, 0u);
 }
 
@@ -1069,10 +1157,10 @@ void bwd_dv(const DevMatrix& m, const float* r, const float* norm, float* out, b
   const dim3 grid(tile_grid(D.ntiles)), block(kBlock);
   if (update)
     hipLaunchKernelGGL((k_bwd_dict<EPT, VK, true>), grid, block, 0, s, m.colptr, m.csc, D.tiles,
-                       D.ntiles, D.dval, D.dlix, D.dict, r, m.rows, norm, out);
+                       D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, r, m.rows, norm, out);
   else
     hipLaunchKernelGGL((k_bwd_dict<EPT, VK, false>), grid, block, 0, s, m.colptr, m.csc, D.tiles,
-                       D.ntiles, D.dval, D.dlix, D.dict, r, m.rows, norm, out);
+                       D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, r, m.rows, norm, out);
 }
 
 template <int EPT>
@@ -1162,20 +1250,25 @@ void build_dict(const std::vector<uint32_t>& ptr, const uint2* el, DictFormat& D
     if (i == nt) break;
     const uint32_t count = ptr[tl[i + 1]] - ptr[tl[i]];
     if (count <= T) acc_a += (count + 63) / 64 * 64;
-    acc_g += nd[i];
+    acc_g += (nd[i] + kBlock - 1) / kBlock * kBlock;  // whole rounds of 4 slots
   }
   if (acc_a + T > 0xFFFFFFFFull || acc_g > 0xFFFFFFFFull)
     raise(MLEM_EINVAL, "matrix too large for the dictionary format");
   free_dict(D);
   D.ntiles = nt;
   D.vk = VK;
-  D.ndict = std::max<uint64_t>(acc_g, 1);
+  D.ndict = std::max<uint64_t>(acc_g, kBlock);
   D.nval = acc_a + T;  // tail: full-tile loads of the last tile stay in bounds
   const uint64_t vbytes = (VK == kF32 ? 4ull : 2ull) * D.nval;
+  const uint64_t nslots = D.ndict / kWave;
+  const uint64_t nmeta = nslots + 4ull * EPT;  // tail: rounded-up rounds of the last tile
   MLEM_HIP(hipMalloc(&D.tiles, sizeof(uint4) * d4.size()));
   MLEM_HIP(hipMalloc(&D.dict, 4ull * D.ndict));
+  MLEM_HIP(hipMalloc(&D.dmeta, 4ull * nmeta));
   MLEM_HIP(hipMalloc(&D.dval, vbytes));
   MLEM_HIP(hipMalloc(&D.dlix, 2ull * D.nval));
+  MLEM_HIP(hipMemsetAsync(D.dict, 0, 4ull * D.ndict, s));
+  MLEM_HIP(hipMemsetAsync(D.dmeta, 0, 4ull * nmeta, s));
   MLEM_HIP(hipMemsetAsync(D.dval, 0, vbytes, s));
   MLEM_HIP(hipMemsetAsync(D.dlix, 0, 2ull * D.nval, s));
   MLEM_HIP(hipMemcpyAsync(D.tiles, d4.data(), sizeof(uint4) * d4.size(), hipMemcpyHostToDevice, s));
@@ -1184,6 +1277,10 @@ void build_dict(const std::vector<uint32_t>& ptr, const uint2* el, DictFormat& D
                        (const uint2*)nullptr, (const uint4*)D.tiles, (uint32_t*)nullptr, D.dict,
                        D.dval, D.dlix);
     check_launch("dictionary fill launch");
+    const uint64_t per = kBlock / kWave;
+    hipLaunchKernelGGL(k_dict_pack, dim3((unsigned)((nslots + per - 1) / per)), dim3(kBlock), 0, s,
+                       D.dict, D.dmeta, nslots);
+    check_launch("dictionary pack launch");
   }
   MLEM_HIP(hipStreamSynchronize(s));
   D.ept = EPT;
@@ -1219,6 +1316,7 @@ void ensure_dict(const std::vector<uint32_t>& ptr, const uint2* el, int ept, int
 void free_dict(DictFormat& d) {
   if (d.tiles) (void)hipFree(d.tiles);
   if (d.dict) (void)hipFree(d.dict);
+  if (d.dmeta) (void)hipFree(d.dmeta);
   if (d.dval) (void)hipFree(d.dval);
   if (d.dlix) (void)hipFree(d.dlix);
   d = DictFormat();
