@@ -566,7 +566,7 @@ __device__ __forceinline__ void load_dict_tile(const DictSlots& ds, __amdgpu_buf
   for (int j = 0; j < R; ++j) {
     const uint32_t m = ds.meta[4 * j + ds.wave];
     const uint32_t w = 256u * j + 64u * ds.wave + ((m >> 31) ? ds.lane : ds.lane >> 1);
-    ix[j] = BufferLoad<uint32_t, 0>::load(ds.words, w * 4u);
+    ix[j] = BufferLoad<uint32_t, kCacheNT>::load(ds.words, w * 4u);  // read once per pass
   }
   __builtin_amdgcn_sched_barrier(0);  // one wait between the index and the gather loads
 #pragma unroll
