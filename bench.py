@@ -257,7 +257,7 @@ def main() -> int:
     dom = "fwd" if fwd_ms >= bwd_ms else "bwd"
     dom_ms = max(fwd_ms, bwd_ms)
     achieved = B[dom] / (dom_ms * 1e-3) / 1e9
-    suffix = {"dict": "_dict", "tiles": "_tiles", "groups": ""}[launch[dom][0]]
+    suffix = {"dict512": "_dict", "dict": "_dict", "tiles": "_tiles", "groups": ""}[launch[dom][0]]
     if args.mode == "exact":
         suffix = ""
     name = {"fwd": f"k_fwd{suffix} (CSR forward projection + fused ratio)",

@@ -300,19 +300,23 @@ MLEM_API void mlem_tape_destroy(mlem_tape* t);
  * FAST-mode kernel family and its parameter, per projection:
  *   MLEM_KERNEL_TILES  param = elements per lane (4, 8 or 16): 256-lane
  *                      blocks stream tiles of 256*param elements made of
- *                      whole rows (columns) and reduce them in LDS (default,
- *                      param 8);
+ *                      whole rows (columns) and reduce them in LDS;
  *   MLEM_KERNEL_GROUPS param = lanes per row (column): 4, 8, 16, 32 or 64.
- *   MLEM_KERNEL_DICT   param = elements per lane (8 or 16): tiles whose
- *                      distinct gather indices are stored as a sorted
- *                      per-tile dictionary (built on the device on first
- *                      use); each distinct x (r) value is gathered once
- *                      into LDS.
+ *   MLEM_KERNEL_DICT   param = elements per lane (8, 16, 24, 32, 48 or 64):
+ *                      256-lane blocks over tiles of at most 256*param
+ *                      elements and 256 whole rows (columns) whose distinct
+ *                      gather indices are stored as a sorted per-tile
+ *                      dictionary (built on the device on first use); each
+ *                      distinct x (r) value is gathered once into LDS.
+ *   MLEM_KERNEL_DICT_WIDE param = elements per lane (16, 24 or 32): the same
+ *                      with 512-lane blocks, tiles of at most 512*param
+ *                      elements and 512 rows (columns) (default, param 32).
  * No reference counterpart (the reference hard-codes 1024-thread blocks,
  * mlem_cuda/versions/version_5.cu:121-126). */
 #define MLEM_KERNEL_GROUPS 0
 #define MLEM_KERNEL_TILES 1
 #define MLEM_KERNEL_DICT 2
+#define MLEM_KERNEL_DICT_WIDE 3
 MLEM_API int mlem_get_launch(const mlem_matrix* m, int* fwd_kernel,
                              int* fwd_param, int* bwd_kernel, int* bwd_param);
 MLEM_API int mlem_set_launch(mlem_matrix* m, int fwd_kernel, int fwd_param,

@@ -146,15 +146,16 @@ class Csr4Matrix:
 
     def launch(self) -> dict:
         """FAST-mode kernels: {"fwd": (family, param), "bwd": (family, param)},
-        family "tiles" (param = elements per lane) or "groups" (lanes/row)."""
+        family "dict512", "dict" or "tiles" (param = elements per lane) or
+        "groups" (lanes/row)."""
         v = [C.c_int() for _ in range(4)]
         check(lib().mlem_get_launch(self.handle, *[C.byref(a) for a in v]))
-        fam = {0: "groups", 1: "tiles", 2: "dict"}
+        fam = {0: "groups", 1: "tiles", 2: "dict", 3: "dict512"}
         return {"fwd": (fam[v[0].value], v[1].value), "bwd": (fam[v[2].value], v[3].value)}
 
     def set_launch(self, fwd, bwd) -> None:
         """fwd, bwd: (family, param) as returned by launch()."""
-        fam = {"groups": 0, "tiles": 1, "dict": 2}
+        fam = {"groups": 0, "tiles": 1, "dict": 2, "dict512": 3}
         check(lib().mlem_set_launch(self.handle, fam[fwd[0]], int(fwd[1]), fam[bwd[0]],
                                     int(bwd[1])))
 

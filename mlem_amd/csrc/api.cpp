@@ -231,6 +231,10 @@ GroupLaunch make_launch(int kind, int param) {
     if (param != 8 && param != 16 && param != 24 && param != 32 && param != 48 && param != 64)
       raise(MLEM_EINVAL, "dictionary tile elements per lane must be 8, 16, 24, 32, 48 or 64");
     g.ept = param;
+  } else if (kind == kDictWide) {
+    if (param != 16 && param != 24 && param != 32)
+      raise(MLEM_EINVAL, "wide dictionary tile elements per lane must be 16, 24 or 32");
+    g.ept = param;
   } else if (kind == kGroups) {
     if (param != 4 && param != 8 && param != 16 && param != 32 && param != 64)
       raise(MLEM_EINVAL, "lanes per row must be 4, 8, 16, 32 or 64");
@@ -244,6 +248,7 @@ GroupLaunch make_launch(int kind, int param) {
 // Candidate FAST-mode configurations of one projection.
 std::vector<GroupLaunch> launch_candidates() {
   std::vector<GroupLaunch> c;
+  for (int e : {16, 24, 32}) c.push_back(make_launch(kDictWide, e));
   for (int e : {8, 16, 24, 32, 48, 64}) c.push_back(make_launch(kDict, e));
   for (int e : {4, 8, 16}) c.push_back(make_launch(kTiles, e));
   for (int g : {8, 16, 32, 64}) c.push_back(make_launch(kGroups, g));
@@ -368,10 +373,11 @@ void finish_open(mlem_matrix* h, unsigned flags) {
   h->norm_seconds = seconds_since(t0);
 
   const double avgRow = m.rows ? (double)m.nnz / m.rows : 1.0;
-  // 12288-element dictionary tiles: the largest that keep three blocks per CU
-  // (LDS-bound); mlem_autotune measures the alternatives on the real matrix.
-  m.fwd = make_launch(kDict, 48);
-  m.bwd = make_launch(kDict, 48);
+  // 16384-element dictionary tiles run by 512-lane blocks, two per CU (LDS-
+  // and register-bound); mlem_autotune measures the alternatives on the real
+  // matrix.
+  m.fwd = make_launch(kDictWide, 32);
+  m.bwd = make_launch(kDictWide, 32);
   m.scatter = make_launch(kGroups, lanes_for(avgRow));
   prepare_tiles(m, m.stream);
   if (env_int("MLEM_AUTOTUNE", 0) > 0) autotune(m);

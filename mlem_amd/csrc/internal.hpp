@@ -52,19 +52,24 @@ int guarded(F&& f) {
   }
 }
 
-// FAST-mode kernel family of a projection.
-enum KernelKind { kGroups = 0, kTiles = 1, kDict = 2 };
+// FAST-mode kernel family of a projection. kDict and kDictWide are the same
+// dictionary tiles run by 256-lane and 512-lane blocks.
+enum KernelKind { kGroups = 0, kTiles = 1, kDict = 2, kDictWide = 3 };
+inline bool is_dict(int kind) { return kind == kDict || kind == kDictWide; }
+inline int dict_lanes(int kind) { return kind == kDictWide ? 512 : 256; }
 
 // Dictionary-tile storage of one projection (built on the device at load
-// time for the elements-per-lane setting `ept`; 0 = not built).
+// time for `nb` lanes per block and `ept` elements per lane: tiles of at most
+// nb*ept elements and nb segments; 0 = not built).
 struct DictFormat {
+  int nb = 0;
   int ept = 0;
   int vk = 0;                 // value storage (ValueKind): fp32, fp16, bf16
   uint32_t ntiles = 0;
   uint4* tiles = nullptr;     // [ntiles+1] {first segment, first element,
                               //  aligned element start, first dict word}
   uint32_t* dict = nullptr;   // distinct gather indices, sorted per tile, in
-                              //  64-entry slots (256 words per 256 entries)
+                              //  64-word slots of 64 entries, nb/64 per round
   uint32_t* dmeta = nullptr;  // per slot: first index | 0x80000000 if 32-bit
   void* dval = nullptr;       // element values, tile-aligned (4 or 2 bytes)
   uint16_t* dlix = nullptr;   // element positions in the tile dictionary

@@ -115,10 +115,11 @@ __device__ __forceinline__ float seg_dot(const uint2* __restrict__ a, const floa
 // Deterministic FP64 block + grid reduction of (ll, tot). Every block writes
 // its partial; the block that finishes last sums all partials in a fixed
 // order and appends the result to the ring.
+template <int NB = kBlock>
 __device__ void stats_finish(double ll, double tot, double2* __restrict__ partials,
                              unsigned* __restrict__ counter, double2* __restrict__ results,
                              unsigned ring) {
-  __shared__ double s_ll[kBlock / kWave], s_tot[kBlock / kWave];
+  __shared__ double s_ll[NB / kWave], s_tot[NB / kWave];
   __shared__ int s_last;
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) {
@@ -133,7 +134,7 @@ __device__ void stats_finish(double ll, double tot, double2* __restrict__ partia
   __syncthreads();
   if (threadIdx.x == 0) {
     double bl = 0.0, bt = 0.0;
-    for (int w = 0; w < kBlock / kWave; ++w) {
+    for (int w = 0; w < NB / kWave; ++w) {
       bl += s_ll[w];
       bt += s_tot[w];
     }
@@ -147,7 +148,7 @@ __device__ void stats_finish(double ll, double tot, double2* __restrict__ partia
   if (!s_last) return;
   __threadfence();
   double pl = 0.0, pt = 0.0;
-  for (unsigned i = threadIdx.x; i < gridDim.x; i += kBlock) {
+  for (unsigned i = threadIdx.x; i < gridDim.x; i += NB) {
     pl += __hip_atomic_load(&partials[i].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     pt += __hip_atomic_load(&partials[i].y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
@@ -164,7 +165,7 @@ __device__ void stats_finish(double ll, double tot, double2* __restrict__ partia
   __syncthreads();
   if (threadIdx.x == 0) {
     double bl = 0.0, bt = 0.0;
-    for (int w = 0; w < kBlock / kWave; ++w) {
+    for (int w = 0; w < NB / kWave; ++w) {
       bl += s_ll[w];
       bt += s_tot[w];
     }
@@ -415,29 +416,27 @@ __global__ __launch_bounds__(kBlock) void k_bwd_tiles(const uint32_t* __restrict
 // a 64-element boundary). At run time a block gathers each distinct entry
 // once, with sorted (coalescing) addresses, into LDS; elements read their
 // operand from LDS. Tiles of one long segment use the 8-byte records and
-// direct gathers.
-struct DictTileDesc {
-  uint32_t s0, e0, a0, g0;  // first segment, first element, aligned start, first dict entry
-};
+// direct gathers. Blocks have NB lanes (256 or 512); a tile holds at most NB
+// whole segments and NB*EPT elements.
 
-template <int EPT>
+template <int NB, int EPT>
 __device__ __forceinline__ float long_segment_sum(const uint2* __restrict__ el,
                                                   const float* __restrict__ v, uint32_t e0,
                                                   uint32_t e1, float* red) {
-  constexpr uint32_t T = kBlock * EPT;
+  constexpr uint32_t T = NB * EPT;
   const uint32_t t = threadIdx.x;
   float acc = 0.f;
   for (uint32_t base = e0; base < e1; base += T) {
     u32x2 e[EPT];
     float g[EPT];
 #pragma unroll
-    for (int j = 0; j < EPT; ++j) e[j] = ld_stream(el + min(base + t + j * kBlock, e1 - 1));
+    for (int j = 0; j < EPT; ++j) e[j] = ld_stream(el + min(base + t + j * NB, e1 - 1));
 #pragma unroll
     for (int j = 0; j < EPT; ++j) g[j] = v[e[j].x];
 #pragma unroll
     for (int j = 0; j < EPT; ++j) {
       const float p = __uint_as_float(e[j].y) * g[j];
-      acc += (base + t + j * kBlock < e1) ? p : 0.f;
+      acc += (base + t + j * NB < e1) ? p : 0.f;
     }
   }
 #pragma unroll
@@ -445,7 +444,7 @@ __device__ __forceinline__ float long_segment_sum(const uint2* __restrict__ el,
   if (t % kWave == 0) red[t / kWave] = acc;
   __syncthreads();
   float sum = 0.f;
-  for (int w = 0; w < kBlock / kWave; ++w) sum += red[w];
+  for (int w = 0; w < NB / kWave; ++w) sum += red[w];
   return sum;
 }
 
@@ -506,12 +505,9 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, 
                                            0x00020000);
 }
 constexpr int kCacheNT = 2;  // cache-policy bit: non-temporal (streamed once)
-// Dictionary tiles: issue the element stream before the dictionary gathers.
-constexpr bool kStreamFirst = true;
-#ifndef MLEM_DICT_ALIAS_MIN_EPT
-#define MLEM_DICT_ALIAS_MIN_EPT 24
-#endif
-constexpr int kDictAliasMinEPT = MLEM_DICT_ALIAS_MIN_EPT;
+// Dictionary tiles of at least this many elements store the products over the
+// gathered dictionary values in LDS (half the LDS, one more barrier).
+constexpr uint32_t kDictAliasMin = 6144;
 
 template <class W, int AUX>
 struct BufferLoad;
@@ -534,8 +530,8 @@ struct BufferLoad<u32x4, AUX> {
   }
 };
 
-// The dictionary of one tile as stored: round j (256 entries) is four
-// 64-word slots, one per wave; slot s of the tile holds 64 32-bit indices, or,
+// The dictionary of one tile as stored: round j (one entry per lane of the
+// block) is one 64-word slot per wave; slot s of the tile holds 64 32-bit indices, or,
 // when meta[s] has bit 31 clear, 64 16-bit offsets from meta[s] in its first
 // 32 words. Entries past the dictionary resolve to a valid index.
 struct DictSlots {
@@ -544,71 +540,63 @@ struct DictSlots {
   uint32_t wave, lane;           // wave index (uniform) and lane within it
 };
 
-// Loads of one dictionary tile: R rounds of 256 dictionary indices, the R
-// gathers they address, then the H element packs (values and dictionary
-// positions) that stay in registers; the gathered values go to V last, so
-// the element stream is in flight while the gathers complete.
-template <int R, int H, class W, class L>
+// Loads of one dictionary tile: first the H element packs (values and
+// dictionary positions) that stay in registers, then R rounds of NB
+// dictionary indices, then the R gathers they address; the gathered values
+// go to V last, so the element stream is in flight while the gathers
+// complete. (Issuing the indices before the stream, so that the gathers need
+// not wait for it, measured 4-15% slower: more registers, fewer waves.)
+template <int NB, int R, int H, class W, class L>
 __device__ __forceinline__ void load_dict_tile(const DictSlots& ds, __amdgpu_buffer_rsrc_t rv,
                                                __amdgpu_buffer_rsrc_t rval,
                                                __amdgpu_buffer_rsrc_t rlix, uint32_t t, float* V,
                                                W (&val)[H], L (&li)[H]) {
-  if constexpr (kStreamFirst) {
 #pragma unroll
-    for (int j = 0; j < H; ++j) {
-      val[j] = BufferLoad<W, kCacheNT>::load(rval, (t + j * kBlock) * (uint32_t)sizeof(W));
-      li[j] = BufferLoad<L, kCacheNT>::load(rlix, (t + j * kBlock) * (uint32_t)sizeof(L));
-    }
-    __builtin_amdgcn_sched_barrier(0);
+  for (int j = 0; j < H; ++j) {
+    val[j] = BufferLoad<W, kCacheNT>::load(rval, (t + j * NB) * (uint32_t)sizeof(W));
+    li[j] = BufferLoad<L, kCacheNT>::load(rlix, (t + j * NB) * (uint32_t)sizeof(L));
   }
+  __builtin_amdgcn_sched_barrier(0);
   uint32_t ix[R];
 #pragma unroll
   for (int j = 0; j < R; ++j) {
-    const uint32_t m = ds.meta[4 * j + ds.wave];
-    const uint32_t w = 256u * j + 64u * ds.wave + ((m >> 31) ? ds.lane : ds.lane >> 1);
+    const uint32_t m = ds.meta[(NB / kWave) * j + ds.wave];
+    const uint32_t w = NB * j + kWave * ds.wave + ((m >> 31) ? ds.lane : ds.lane >> 1);
     ix[j] = BufferLoad<uint32_t, kCacheNT>::load(ds.words, w * 4u);  // read once per pass
   }
   __builtin_amdgcn_sched_barrier(0);  // one wait between the index and the gather loads
 #pragma unroll
   for (int j = 0; j < R; ++j) {
-    const uint32_t m = ds.meta[4 * j + ds.wave];
+    const uint32_t m = ds.meta[(NB / kWave) * j + ds.wave];
     if (!(m >> 31)) ix[j] = m + ((ix[j] >> (16 * (ds.lane & 1))) & 0xFFFFu);
   }
   float g[R];
 #pragma unroll
   for (int j = 0; j < R; ++j) g[j] = __uint_as_float(BufferLoad<uint32_t, 0>::load(rv, ix[j] * 4u));
   __builtin_amdgcn_sched_barrier(0);
-  if constexpr (!kStreamFirst) {
 #pragma unroll
-    for (int j = 0; j < H; ++j) {
-      val[j] = BufferLoad<W, kCacheNT>::load(rval, (t + j * kBlock) * (uint32_t)sizeof(W));
-      li[j] = BufferLoad<L, kCacheNT>::load(rlix, (t + j * kBlock) * (uint32_t)sizeof(L));
-    }
-    __builtin_amdgcn_sched_barrier(0);
-  }
-#pragma unroll
-  for (int j = 0; j < R; ++j) V[t + j * kBlock] = g[j];
+  for (int j = 0; j < R; ++j) V[t + j * NB] = g[j];
 }
 
 // load_dict_tile with the smallest multiple of 4 rounds >= `rounds` (at most
 // EPT): a chain of block-uniform branches over fully unrolled variants.
-template <int R, int EPT, int H, class W, class L>
+template <int NB, int R, int EPT, int H, class W, class L>
 __device__ __forceinline__ void load_dict_rounds(uint32_t rounds, const DictSlots& ds,
                                                  __amdgpu_buffer_rsrc_t rv,
                                                  __amdgpu_buffer_rsrc_t rval,
                                                  __amdgpu_buffer_rsrc_t rlix, uint32_t t, float* V,
                                                  W (&val)[H], L (&li)[H]) {
   if constexpr (R >= EPT) {
-    load_dict_tile<EPT>(ds, rv, rval, rlix, t, V, val, li);
+    load_dict_tile<NB, EPT>(ds, rv, rval, rlix, t, V, val, li);
   } else {
     if (rounds <= (uint32_t)R)
-      load_dict_tile<R>(ds, rv, rval, rlix, t, V, val, li);
+      load_dict_tile<NB, R>(ds, rv, rval, rlix, t, V, val, li);
     else
-      load_dict_rounds<R + 4, EPT>(rounds, ds, rv, rval, rlix, t, V, val, li);
+      load_dict_rounds<NB, R + 4, EPT>(rounds, ds, rv, rval, rlix, t, V, val, li);
   }
 }
 
-template <int EPT, int VK, class Epi>
+template <int NB, int EPT, int VK, class Epi>
 __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
                                           const uint2* __restrict__ el,
                                           const uint4* __restrict__ tiles, uint32_t tile,
@@ -624,25 +612,25 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
   using W = typename VP::W;
   using L = typename LP::W;
   using F = float __attribute__((ext_vector_type(NP)));
-  constexpr uint32_t T = kBlock * EPT;
+  constexpr uint32_t T = NB * EPT;
   constexpr int H = EPT / NP;
   constexpr int G = 16;
-  // Large tiles store the products over the dictionary values (half the LDS,
-  // one more barrier); smaller tiles keep two buffers.
-  constexpr bool kAlias = EPT >= kDictAliasMinEPT;
+  // Large tiles store the products over the dictionary values; smaller tiles
+  // keep two buffers.
+  constexpr bool kAlias = T >= kDictAliasMin;
   __shared__ __align__(16) float V[T];                  // gathered dictionary values
   __shared__ __align__(16) float Pbuf[kAlias ? 2 : T];  // element products, tile order
   float* P = kAlias ? V : Pbuf;
-  __shared__ uint32_t S[kBlock + 1];
-  __shared__ float Sum[kBlock];
-  __shared__ float red[kBlock / kWave];
+  __shared__ uint32_t S[NB + 1];
+  __shared__ float Sum[NB];
+  __shared__ float red[NB / kWave];
   const uint32_t t = threadIdx.x;
   const uint4 d0 = tiles[tile], d1 = tiles[tile + 1];
   const uint32_t s0 = d0.x, e0 = d0.y, a0 = d0.z, g0 = d0.w;
   const uint32_t nseg = d1.x - s0, count = d1.y - e0, nwords = d1.w - g0;
   if (t < nseg) epi.prefetch(s0 + t);
   if (count > T) {
-    const float sum = long_segment_sum<EPT>(el, v, e0, d1.y, red);
+    const float sum = long_segment_sum<NB, EPT>(el, v, e0, d1.y, red);
     if (t == 0) epi.finish(s0, sum);
     return;
   }
@@ -656,20 +644,20 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
     const auto rlix = buffer_rsrc(dlix + a0, T * 2);
     // Rounds past the tile's own read words past its buffer (zeros) under
     // the next tile's slot descriptors (the descriptor array is padded by
-    // 4*EPT zeros), which resolve to valid indices as well.
+    // EPT rounds of zeros), which resolve to valid indices as well.
     const DictSlots ds{buffer_rsrc(dict + g0, nwords * 4), dmeta + (g0 >> 6),
                        (uint32_t)__builtin_amdgcn_readfirstlane(t / kWave), t % kWave};
     const auto rv = buffer_rsrc(v, nv * 4);
-    // Gather only as many 256-entry rounds as the dictionary needs, rounded
+    // Gather only as many NB-entry rounds as the dictionary needs, rounded
     // up to a multiple of 4 (block-uniform branch; each variant issues all
     // its loads before the first wait).
-    const uint32_t rounds = __builtin_amdgcn_readfirstlane(nwords / kBlock);
-    load_dict_rounds<4, EPT>(rounds, ds, rv, rval, rlix, t, V, val, li);
+    const uint32_t rounds = __builtin_amdgcn_readfirstlane(nwords / NB);
+    load_dict_rounds<NB, 4, EPT>(rounds, ds, rv, rval, rlix, t, V, val, li);
   }
   if (t < nseg) S[t + 1] = send;
   if (t == 0) S[0] = 0;
   __syncthreads();
-  // Products of pack j of this lane are elements NP*(t + j*kBlock) + q.
+  // Products of pack j of this lane are elements NP*(t + j*NB) + q.
   if constexpr (kAlias) {
     F pr[H];
     if (count > 0) {
@@ -682,7 +670,7 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
     if (count > 0) {
 #pragma unroll
       for (int j = 0; j < H; ++j)  // one NP*4-byte store per pack: no bank conflicts
-        reinterpret_cast<F*>(P)[t + j * kBlock] = pr[j];
+        reinterpret_cast<F*>(P)[t + j * NB] = pr[j];
     }
   } else if (count > 0) {
 #pragma unroll
@@ -690,12 +678,12 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
       F pr;
 #pragma unroll
       for (int q = 0; q < NP; ++q) pr[q] = VP::get(val[j], q) * V[LP::get(li[j], q)];
-      reinterpret_cast<F*>(P)[t + j * kBlock] = pr;
+      reinterpret_cast<F*>(P)[t + j * NB] = pr;
     }
   }
   __syncthreads();
   const uint32_t lane = t % G, grp = t / G;
-  for (uint32_t sg = grp; sg < nseg; sg += kBlock / G) {
+  for (uint32_t sg = grp; sg < nseg; sg += NB / G) {
     const uint32_t b = S[sg], end = S[sg + 1];
     // aligned 16-byte reads of the segment's products; the first and last
     // quads are masked to [b, end)
@@ -715,8 +703,8 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
   if (t < nseg) epi.finish(s0 + t, Sum[t]);
 }
 
-template <int EPT, int VK, bool RATIO, bool STATS>
-__global__ __launch_bounds__(kBlock) void k_fwd_dict(
+template <int NB, int EPT, int VK, bool RATIO, bool STATS>
+__global__ __launch_bounds__(NB) void k_fwd_dict(
     const uint32_t* __restrict__ rowptr, const uint2* __restrict__ csr,
     const uint4* __restrict__ tiles, uint32_t ntiles, const void* __restrict__ dval,
     const uint16_t* __restrict__ dlix, const uint32_t* __restrict__ dict,
@@ -727,12 +715,12 @@ __global__ __launch_bounds__(kBlock) void k_fwd_dict(
   const uint32_t tile = logical_block();
   FwdEpilogue<RATIO, STATS> epi{y, out, rscale};
   if (tile < ntiles)
-    dict_spmv<EPT, VK>(rowptr, csr, tiles, tile, dval, dlix, dict, dmeta, x, cols, epi);
-  if constexpr (STATS) stats_finish(epi.ll, epi.tot, partials, counter, results, ring);
+    dict_spmv<NB, EPT, VK>(rowptr, csr, tiles, tile, dval, dlix, dict, dmeta, x, cols, epi);
+  if constexpr (STATS) stats_finish<NB>(epi.ll, epi.tot, partials, counter, results, ring);
 }
 
-template <int EPT, int VK, bool UPDATE>
-__global__ __launch_bounds__(kBlock) void k_bwd_dict(
+template <int NB, int EPT, int VK, bool UPDATE>
+__global__ __launch_bounds__(NB) void k_bwd_dict(
     const uint32_t* __restrict__ colptr, const uint2* __restrict__ csc,
     const uint4* __restrict__ tiles, uint32_t ntiles, const void* __restrict__ dval,
     const uint16_t* __restrict__ dlix, const uint32_t* __restrict__ dict,
@@ -742,7 +730,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd_dict(
   const uint32_t tile = logical_block();
   if (tile >= ntiles) return;
   BwdEpilogue<UPDATE> epi{norm, out};
-  dict_spmv<EPT, VK>(colptr, csc, tiles, tile, dval, dlix, dict, dmeta, r, rows, epi);
+  dict_spmv<NB, EPT, VK>(colptr, csc, tiles, tile, dval, dlix, dict, dmeta, r, rows, epi);
 }
 
 // Packs every 64-entry dictionary slot in place, one wave per slot: when the
@@ -1116,69 +1104,73 @@ void check_launch(const char* what) {
   if (e != hipSuccess) raise(MLEM_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-template <int EPT, int VK>
+template <int NB, int EPT, int VK>
 void fwd_dv(const DevMatrix& m, const float* x, const int32_t* y, float* out, bool ratio,
             const StatsState* st, hipStream_t s) {
   const DictFormat& D = m.dfwd;
   const bool stats = ratio && st != nullptr;
-  const dim3 grid(tile_grid(D.ntiles)), block(kBlock);
+  const dim3 grid(tile_grid(D.ntiles)), block(NB);
   if (stats && grid.x > st->max_blocks) raise(MLEM_ESTATE, "statistics buffer too small for launch");
   if (stats)
-    hipLaunchKernelGGL((k_fwd_dict<EPT, VK, true, true>), grid, block, 0, s, m.rowptr, m.csr,
+    hipLaunchKernelGGL((k_fwd_dict<NB, EPT, VK, true, true>), grid, block, 0, s, m.rowptr, m.csr,
                        D.tiles, D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, x, m.cols, y, out, m.rscale,
                        st->partials, st->counter, st->results, st->ring);
   else if (ratio)
-    hipLaunchKernelGGL((k_fwd_dict<EPT, VK, true, false>), grid, block, 0, s, m.rowptr, m.csr,
+    hipLaunchKernelGGL((k_fwd_dict<NB, EPT, VK, true, false>), grid, block, 0, s, m.rowptr, m.csr,
                        D.tiles, D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, x, m.cols, y, out, m.rscale, nullptr,
                        nullptr, nullptr, 0u);
   else
-    hipLaunchKernelGGL((k_fwd_dict<EPT, VK, false, false>), grid, block, 0, s, m.rowptr, m.csr,
+    hipLaunchKernelGGL((k_fwd_dict<NB, EPT, VK, false, false>), grid, block, 0, s, m.rowptr, m.csr,
                        D.tiles, D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, x, m.cols, y, out, m.rscale, nullptr,
                        nullptr, nullptr, 0u);
 }
 
-template <int EPT>
+template <int NB, int EPT>
 void fwd_d(const DevMatrix& m, const float* x, const int32_t* y, float* out, bool ratio,
            const StatsState* st, hipStream_t s) {
   const DictFormat& D = m.dfwd;
-  if (D.ept != EPT || !D.tiles) raise(MLEM_ESTATE, "forward dictionary not prepared");
+  if (D.nb != NB || D.ept != EPT || !D.tiles)
+    raise(MLEM_ESTATE, "forward dictionary not prepared");
   if (m.cols >= (1u << 30)) raise(MLEM_EINVAL, "dictionary kernels need cols < 2^30");
   switch (D.vk) {
-    case kF32: fwd_dv<EPT, kF32>(m, x, y, out, ratio, st, s); break;
-    case kF16: fwd_dv<EPT, kF16>(m, x, y, out, ratio, st, s); break;
-    default: fwd_dv<EPT, kBF16>(m, x, y, out, ratio, st, s); break;
+    case kF32: fwd_dv<NB, EPT, kF32>(m, x, y, out, ratio, st, s); break;
+    case kF16: fwd_dv<NB, EPT, kF16>(m, x, y, out, ratio, st, s); break;
+    default: fwd_dv<NB, EPT, kBF16>(m, x, y, out, ratio, st, s); break;
   }
 }
 
-template <int EPT, int VK>
+template <int NB, int EPT, int VK>
 void bwd_dv(const DevMatrix& m, const float* r, const float* norm, float* out, bool update,
             hipStream_t s) {
   const DictFormat& D = m.dbwd;
-  const dim3 grid(tile_grid(D.ntiles)), block(kBlock);
+  const dim3 grid(tile_grid(D.ntiles)), block(NB);
   if (update)
-    hipLaunchKernelGGL((k_bwd_dict<EPT, VK, true>), grid, block, 0, s, m.colptr, m.csc, D.tiles,
+    hipLaunchKernelGGL((k_bwd_dict<NB, EPT, VK, true>), grid, block, 0, s, m.colptr, m.csc, D.tiles,
                        D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, r, m.rows, norm, out);
   else
-    hipLaunchKernelGGL((k_bwd_dict<EPT, VK, false>), grid, block, 0, s, m.colptr, m.csc, D.tiles,
+    hipLaunchKernelGGL((k_bwd_dict<NB, EPT, VK, false>), grid, block, 0, s, m.colptr, m.csc, D.tiles,
                        D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, r, m.rows, norm, out);
 }
 
-template <int EPT>
+template <int NB, int EPT>
 void bwd_d(const DevMatrix& m, const float* r, const float* norm, float* out, bool update,
            hipStream_t s) {
   const DictFormat& D = m.dbwd;
-  if (D.ept != EPT || !D.tiles) raise(MLEM_ESTATE, "backward dictionary not prepared");
+  if (D.nb != NB || D.ept != EPT || !D.tiles)
+    raise(MLEM_ESTATE, "backward dictionary not prepared");
   if (m.rows >= (1u << 30)) raise(MLEM_EINVAL, "dictionary kernels need rows < 2^30");
   switch (D.vk) {
-    case kF32: bwd_dv<EPT, kF32>(m, r, norm, out, update, s); break;
-    case kF16: bwd_dv<EPT, kF16>(m, r, norm, out, update, s); break;
-    default: bwd_dv<EPT, kBF16>(m, r, norm, out, update, s); break;
+    case kF32: bwd_dv<NB, EPT, kF32>(m, r, norm, out, update, s); break;
+    case kF16: bwd_dv<NB, EPT, kF16>(m, r, norm, out, update, s); break;
+    default: bwd_dv<NB, EPT, kBF16>(m, r, norm, out, update, s); break;
   }
 }
 
 // Greedy partition of segments [0, n) into tiles of at most T elements and
-// 256 segments (one per lane); a segment longer than T forms a tile of its own.
-std::vector<uint32_t> partition_tiles(const std::vector<uint32_t>& ptr, uint32_t T) {
+// `maxseg` segments (one per lane); a segment longer than T forms a tile of
+// its own.
+std::vector<uint32_t> partition_tiles(const std::vector<uint32_t>& ptr, uint32_t T,
+                                      uint32_t maxseg = kBlock) {
   const uint32_t n = (uint32_t)ptr.size() - 1;
   std::vector<uint32_t> tiles{0};
   uint32_t s = 0;
@@ -1188,7 +1180,7 @@ std::vector<uint32_t> partition_tiles(const std::vector<uint32_t>& ptr, uint32_t
       e = s + 1;
     } else {
       uint64_t cnt = 0;
-      while (e < n && e - s < (uint32_t)kBlock && cnt + (ptr[e + 1] - ptr[e]) <= T) {
+      while (e < n && e - s < maxseg && cnt + (ptr[e + 1] - ptr[e]) <= T) {
         cnt += ptr[e + 1] - ptr[e];
         ++e;
       }
@@ -1218,10 +1210,14 @@ void upload_tiles(const std::vector<uint32_t>& ptr, int ept, uint2*& dev, uint32
 
 // Build the dictionary format of one projection from its device-order
 // segments (host copy of the pointers, device 8-byte records).
-template <int EPT, int VK>
+// The format depends on the tile size T = NB*EPT and the segment limit NB;
+// the build kernel runs 256-lane blocks over the same tiles.
+template <int NB, int EPT, int VK>
 void build_dict(const std::vector<uint32_t>& ptr, const uint2* el, DictFormat& D, hipStream_t s) {
-  constexpr uint32_t T = kBlock * EPT;
-  const std::vector<uint32_t> tl = partition_tiles(ptr, T);
+  constexpr uint32_t T = NB * EPT;
+  static_assert(T % kBlock == 0, "build blocks cover whole tiles");
+  constexpr int EB = T / kBlock;  // elements per build lane
+  const std::vector<uint32_t> tl = partition_tiles(ptr, T, NB);
   const uint32_t nt = (uint32_t)tl.size() - 1;
   std::vector<uint2> d2(tl.size());
   for (size_t i = 0; i < tl.size(); ++i) d2[i] = make_uint2(tl[i], ptr[tl[i]]);
@@ -1236,7 +1232,7 @@ void build_dict(const std::vector<uint32_t>& ptr, const uint2* el, DictFormat& D
   MLEM_HIP(hipMemcpyAsync(t2.p, d2.data(), sizeof(uint2) * d2.size(), hipMemcpyHostToDevice, s));
   std::vector<uint32_t> nd(nt);
   if (nt) {
-    hipLaunchKernelGGL((k_dict_build<EPT, VK>), dim3(nt), dim3(kBlock), 0, s, (const uint2*)el,
+    hipLaunchKernelGGL((k_dict_build<EB, VK>), dim3(nt), dim3(kBlock), 0, s, (const uint2*)el,
                        (const uint2*)t2.p, (const uint4*)nullptr, (uint32_t*)ndb.p,
                        (uint32_t*)nullptr, (void*)nullptr, (uint16_t*)nullptr);
     check_launch("dictionary count launch");
@@ -1250,18 +1246,18 @@ void build_dict(const std::vector<uint32_t>& ptr, const uint2* el, DictFormat& D
     if (i == nt) break;
     const uint32_t count = ptr[tl[i + 1]] - ptr[tl[i]];
     if (count <= T) acc_a += (count + 63) / 64 * 64;
-    acc_g += (nd[i] + kBlock - 1) / kBlock * kBlock;  // whole rounds of 4 slots
+    acc_g += (nd[i] + NB - 1) / NB * NB;  // whole rounds of one slot per wave
   }
   if (acc_a + T > 0xFFFFFFFFull || acc_g > 0xFFFFFFFFull)
     raise(MLEM_EINVAL, "matrix too large for the dictionary format");
   free_dict(D);
   D.ntiles = nt;
   D.vk = VK;
-  D.ndict = std::max<uint64_t>(acc_g, kBlock);
+  D.ndict = std::max<uint64_t>(acc_g, NB);
   D.nval = acc_a + T;  // tail: full-tile loads of the last tile stay in bounds
   const uint64_t vbytes = (VK == kF32 ? 4ull : 2ull) * D.nval;
   const uint64_t nslots = D.ndict / kWave;
-  const uint64_t nmeta = nslots + 4ull * EPT;  // tail: rounded-up rounds of the last tile
+  const uint64_t nmeta = nslots + (uint64_t)(NB / kWave) * EPT;  // tail: rounded-up rounds
   MLEM_HIP(hipMalloc(&D.tiles, sizeof(uint4) * d4.size()));
   MLEM_HIP(hipMalloc(&D.dict, 4ull * D.ndict));
   MLEM_HIP(hipMalloc(&D.dmeta, 4ull * nmeta));
@@ -1273,7 +1269,7 @@ void build_dict(const std::vector<uint32_t>& ptr, const uint2* el, DictFormat& D
   MLEM_HIP(hipMemsetAsync(D.dlix, 0, 2ull * D.nval, s));
   MLEM_HIP(hipMemcpyAsync(D.tiles, d4.data(), sizeof(uint4) * d4.size(), hipMemcpyHostToDevice, s));
   if (nt) {
-    hipLaunchKernelGGL((k_dict_build<EPT, VK>), dim3(nt), dim3(kBlock), 0, s, (const uint2*)el,
+    hipLaunchKernelGGL((k_dict_build<EB, VK>), dim3(nt), dim3(kBlock), 0, s, (const uint2*)el,
                        (const uint2*)nullptr, (const uint4*)D.tiles, (uint32_t*)nullptr, D.dict,
                        D.dval, D.dlix);
     check_launch("dictionary fill launch");
@@ -1283,30 +1279,34 @@ void build_dict(const std::vector<uint32_t>& ptr, const uint2* el, DictFormat& D
     check_launch("dictionary pack launch");
   }
   MLEM_HIP(hipStreamSynchronize(s));
+  D.nb = NB;
   D.ept = EPT;
 }
 
 template <int VK>
-void build_dict_vk(const std::vector<uint32_t>& ptr, const uint2* el, int ept, DictFormat& D,
-                   hipStream_t s) {
-  switch (ept) {
-    case 8: build_dict<8, VK>(ptr, el, D, s); break;
-    case 16: build_dict<16, VK>(ptr, el, D, s); break;
-    case 24: build_dict<24, VK>(ptr, el, D, s); break;
-    case 32: build_dict<32, VK>(ptr, el, D, s); break;
-    case 48: build_dict<48, VK>(ptr, el, D, s); break;
-    case 64: build_dict<64, VK>(ptr, el, D, s); break;
+void build_dict_vk(const std::vector<uint32_t>& ptr, const uint2* el, int nb, int ept,
+                   DictFormat& D, hipStream_t s) {
+  switch (nb * 1000 + ept) {
+    case 256008: build_dict<256, 8, VK>(ptr, el, D, s); break;
+    case 256016: build_dict<256, 16, VK>(ptr, el, D, s); break;
+    case 256024: build_dict<256, 24, VK>(ptr, el, D, s); break;
+    case 256032: build_dict<256, 32, VK>(ptr, el, D, s); break;
+    case 256048: build_dict<256, 48, VK>(ptr, el, D, s); break;
+    case 256064: build_dict<256, 64, VK>(ptr, el, D, s); break;
+    case 512016: build_dict<512, 16, VK>(ptr, el, D, s); break;
+    case 512024: build_dict<512, 24, VK>(ptr, el, D, s); break;
+    case 512032: build_dict<512, 32, VK>(ptr, el, D, s); break;
     default: raise(MLEM_EINVAL, "unsupported dictionary tile size");
   }
 }
 
-void ensure_dict(const std::vector<uint32_t>& ptr, const uint2* el, int ept, int vk,
+void ensure_dict(const std::vector<uint32_t>& ptr, const uint2* el, int nb, int ept, int vk,
                  DictFormat& D, hipStream_t s) {
-  if (D.ept == ept && D.vk == vk && D.tiles) return;
+  if (D.nb == nb && D.ept == ept && D.vk == vk && D.tiles) return;
   switch (vk) {
-    case kF32: build_dict_vk<kF32>(ptr, el, ept, D, s); break;
-    case kF16: build_dict_vk<kF16>(ptr, el, ept, D, s); break;
-    case kBF16: build_dict_vk<kBF16>(ptr, el, ept, D, s); break;
+    case kF32: build_dict_vk<kF32>(ptr, el, nb, ept, D, s); break;
+    case kF16: build_dict_vk<kF16>(ptr, el, nb, ept, D, s); break;
+    case kBF16: build_dict_vk<kBF16>(ptr, el, nb, ept, D, s); break;
     default: raise(MLEM_EINVAL, "unsupported value storage");
   }
 }
@@ -1339,29 +1339,41 @@ void prepare_tiles(DevMatrix& m, hipStream_t s) {
     upload_tiles(m.h_rowptr, m.fwd.ept, m.tiles_fwd, m.ntiles_fwd, m.tiles_fwd_ept, s);
   if (m.bwd.kind == kTiles && m.csc)
     upload_tiles(m.h_colptr, m.bwd.ept, m.tiles_bwd, m.ntiles_bwd, m.tiles_bwd_ept, s);
-  if (m.fwd.kind == kDict) ensure_dict(m.h_rowptr, m.csr, m.fwd.ept, m.vk, m.dfwd, s);
-  if (m.bwd.kind == kDict && m.csc) ensure_dict(m.h_colptr, m.csc, m.bwd.ept, m.vk, m.dbwd, s);
+  if (is_dict(m.fwd.kind))
+    ensure_dict(m.h_rowptr, m.csr, dict_lanes(m.fwd.kind), m.fwd.ept, m.vk, m.dfwd, s);
+  if (is_dict(m.bwd.kind) && m.csc)
+    ensure_dict(m.h_colptr, m.csc, dict_lanes(m.bwd.kind), m.bwd.ept, m.vk, m.dbwd, s);
 }
 
+// Blocks of the statistics-producing forward launch: the larger of the FAST
+// launch configured now and the EXACT one (one lane per row).
 unsigned stats_max_blocks(const DevMatrix& m) {
-  if (m.fwd.kind == kDict) return tile_grid(m.dfwd.ntiles);
-  if (m.fwd.kind == kTiles) return tile_grid(m.ntiles_fwd);
-  const Geometry g = geometry(m.rows, m.fwd.lanes, m.fwd.blocks_per_cu, m.cus, true);
-  return g.grid;
+  unsigned fast;
+  if (is_dict(m.fwd.kind))
+    fast = tile_grid(m.dfwd.ntiles);
+  else if (m.fwd.kind == kTiles)
+    fast = tile_grid(m.ntiles_fwd);
+  else
+    fast = geometry(m.rows, m.fwd.lanes, m.fwd.blocks_per_cu, m.cus, true).grid;
+  const unsigned exact = geometry(m.rows, 1, m.fwd.blocks_per_cu, m.cus, true).grid;
+  return std::max(fast, exact);
 }
 
 void launch_fwd(const DevMatrix& m, int mode, const float* x, const int32_t* y, float* out,
                 bool ratio, const StatsState* stats, hipStream_t s) {
   if (mode == MLEM_MODE_EXACT) {
     fwd_g<1, true>(m, x, y, out, ratio, stats, s);
-  } else if (m.fwd.kind == kDict) {
-    switch (m.fwd.ept) {
-      case 8: fwd_d<8>(m, x, y, out, ratio, stats, s); break;
-      case 16: fwd_d<16>(m, x, y, out, ratio, stats, s); break;
-      case 24: fwd_d<24>(m, x, y, out, ratio, stats, s); break;
-      case 32: fwd_d<32>(m, x, y, out, ratio, stats, s); break;
-      case 48: fwd_d<48>(m, x, y, out, ratio, stats, s); break;
-      case 64: fwd_d<64>(m, x, y, out, ratio, stats, s); break;
+  } else if (is_dict(m.fwd.kind)) {
+    switch (dict_lanes(m.fwd.kind) * 1000 + m.fwd.ept) {
+      case 256008: fwd_d<256, 8>(m, x, y, out, ratio, stats, s); break;
+      case 256016: fwd_d<256, 16>(m, x, y, out, ratio, stats, s); break;
+      case 256024: fwd_d<256, 24>(m, x, y, out, ratio, stats, s); break;
+      case 256032: fwd_d<256, 32>(m, x, y, out, ratio, stats, s); break;
+      case 256048: fwd_d<256, 48>(m, x, y, out, ratio, stats, s); break;
+      case 256064: fwd_d<256, 64>(m, x, y, out, ratio, stats, s); break;
+      case 512016: fwd_d<512, 16>(m, x, y, out, ratio, stats, s); break;
+      case 512024: fwd_d<512, 24>(m, x, y, out, ratio, stats, s); break;
+      case 512032: fwd_d<512, 32>(m, x, y, out, ratio, stats, s); break;
       default: raise(MLEM_EINVAL, "unsupported forward dictionary tile size");
     }
   } else if (m.fwd.kind == kTiles) {
@@ -1389,14 +1401,17 @@ void launch_bwd(const DevMatrix& m, int mode, const float* r, const float* norm,
   if (!m.csc) raise(MLEM_ESTATE, "matrix was opened without a CSC transpose");
   if (mode == MLEM_MODE_EXACT) {
     bwd_g<1, true>(m, r, norm, out, update, s);
-  } else if (m.bwd.kind == kDict) {
-    switch (m.bwd.ept) {
-      case 8: bwd_d<8>(m, r, norm, out, update, s); break;
-      case 16: bwd_d<16>(m, r, norm, out, update, s); break;
-      case 24: bwd_d<24>(m, r, norm, out, update, s); break;
-      case 32: bwd_d<32>(m, r, norm, out, update, s); break;
-      case 48: bwd_d<48>(m, r, norm, out, update, s); break;
-      case 64: bwd_d<64>(m, r, norm, out, update, s); break;
+  } else if (is_dict(m.bwd.kind)) {
+    switch (dict_lanes(m.bwd.kind) * 1000 + m.bwd.ept) {
+      case 256008: bwd_d<256, 8>(m, r, norm, out, update, s); break;
+      case 256016: bwd_d<256, 16>(m, r, norm, out, update, s); break;
+      case 256024: bwd_d<256, 24>(m, r, norm, out, update, s); break;
+      case 256032: bwd_d<256, 32>(m, r, norm, out, update, s); break;
+      case 256048: bwd_d<256, 48>(m, r, norm, out, update, s); break;
+      case 256064: bwd_d<256, 64>(m, r, norm, out, update, s); break;
+      case 512016: bwd_d<512, 16>(m, r, norm, out, update, s); break;
+      case 512024: bwd_d<512, 24>(m, r, norm, out, update, s); break;
+      case 512032: bwd_d<512, 32>(m, r, norm, out, update, s); break;
       default: raise(MLEM_EINVAL, "unsupported backward dictionary tile size");
     }
   } else if (m.bwd.kind == kTiles) {

@@ -188,8 +188,8 @@ def test_edge_cases_empty_rows_and_columns():
 
 
 FAMILIES = [("tiles", 4), ("tiles", 8), ("tiles", 16), ("dict", 8), ("dict", 16),
-            ("dict", 24), ("dict", 32), ("dict", 64), ("groups", 4), ("groups", 16),
-            ("groups", 64)]
+            ("dict", 24), ("dict", 32), ("dict", 48), ("dict", 64), ("dict512", 16),
+            ("dict512", 24), ("dict512", 32), ("groups", 4), ("groups", 16), ("groups", 64)]
 
 
 @pytest.fixture(scope="module")
@@ -245,6 +245,31 @@ def test_small_fast_families(small, family):
     xo = orc.mlem(m.rowidx, m.elements, m.cols, p.y, 20)[0]
     assert rel_linf(x, xo) < TOL_RECON
     A.set_launch(("tiles", 8), ("tiles", 8))
+
+
+@pytest.mark.parametrize("family", [("dict", 8), ("dict", 64), ("dict512", 16), ("dict512", 32)],
+                         ids=lambda f: f"{f[0]}{f[1]}")
+def test_short_rows_fill_tiles_by_segment_count(family):
+    # 5000 rows of 0-6 elements and 3000 columns: dictionary tiles end at their
+    # segment limit (256 or 512 rows), not at their element limit
+    rng = np.random.default_rng(21)
+    cols = 3000
+    lens = rng.integers(0, 7, 5000)
+    rows = [np.sort(rng.choice(cols, L, replace=False)) for L in lens]
+    ends = np.cumsum(lens).astype(np.uint64)
+    el = np.zeros(int(ends[-1]), dtype=M.ELEMENT_DTYPE)
+    el["column"] = np.concatenate(rows).astype(np.uint32)
+    el["value"] = rng.uniform(0.1, 1.7, el.size).astype(np.float32)
+    A = M.Csr4Matrix.from_arrays(len(rows), cols, ends, el)
+    A.set_launch(family, family)
+    x = rng.uniform(0.1, 3, cols).astype(np.float32)
+    r = rng.uniform(0.1, 3, A.rows()).astype(np.float32)
+    assert rel_linf(M.calcFwProj(A, x, "fast"), orc.f64_fwproj(ends, el, x)) < TOL_PROJ
+    assert rel_linf(M.calcBkProj(A, r, "fast"), orc.f64_bkproj(ends, el, r, cols)) < TOL_PROJ
+    y = rng.integers(0, 50, A.rows()).astype(np.int32)
+    xg = M.mlem(A, y, None, 6, mode="fast")
+    assert rel_linf(xg, orc.mlem(ends, el, cols, y, 6)[0]) < 1e-5
+    A.close()
 
 
 def test_layout_with_partial_bricks():

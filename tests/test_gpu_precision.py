@@ -97,7 +97,8 @@ def test_recon_vs_fp64_dequantized(qtiny, mode):
     assert rel_linf(ft, ft64) < 1e-6
 
 
-@pytest.mark.parametrize("family", [("dict", 16), ("dict", 64), ("tiles", 8), ("groups", 16)],
+@pytest.mark.parametrize("family", [("dict", 16), ("dict", 64), ("dict512", 32), ("tiles", 8),
+                                    ("groups", 16)],
                          ids=lambda f: f"{f[0]}{f[1]}")
 @pytest.mark.parametrize("kind", KINDS)
 def test_small_families(kind, family):
