@@ -1,0 +1,123 @@
+"""Parity at the benchmark sizes, through checks that hold at any size.
+
+pet: 1,327,104 lines x 786,432 voxels, 168,540,822 nonzeros (the iterations/s
+workload); large: 4,194,304 x 2,359,296, 789.8M nonzeros. EXACT iterates and
+projections must equal the oracle's fp32 restatement of seqcsr4mlem bit for
+bit; FAST results are held to the FP64 oracle and to identities of MLEM:
+count preservation sum_j n_j x_j = sum_i y_i after every update (every row of
+these matrices has elements, so f_i > 0), the forward total sum_i f_i = n . x,
+adjointness <A x, r> = <x, A^T r>, and a non-decreasing log-likelihood.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import rel_linf
+import mlem_amd as M
+from mlem_amd import synth
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+TOL_PROJ = 2e-6   # FAST projections vs FP64, elementwise relative
+TOL_RECON = 1e-4  # FAST reconstructions vs FP64, elementwise relative
+
+
+def _open(p):
+    m = p.matrix
+    return M.Csr4Matrix.from_arrays(m.rows, m.cols, m.rowidx, m.elements, grid=m.grid)
+
+
+@pytest.fixture(scope="module")
+def pet():
+    p = synth.make_problem("pet")
+    print(f"\n[fullsize] pet: {p.matrix.rows} x {p.matrix.cols}, {p.matrix.nnz} nnz", flush=True)
+    A = _open(p)
+    yield p, A
+    A.close()
+
+
+def test_pet_exact_iterates_bitwise(pet):
+    p, A = pet
+    m = p.matrix
+    x = M.mlem(A, p.y, None, 2, mode="exact")
+    assert np.array_equal(x, orc.mlem(m.rowidx, m.elements, m.cols, p.y, 2)[0])
+
+
+def test_pet_projections(pet):
+    p, A = pet
+    m = p.matrix
+    rng = np.random.default_rng(11)
+    x = rng.uniform(0.1, 2.0, m.cols).astype(np.float32)
+    r = rng.uniform(0.1, 2.0, m.rows).astype(np.float32)
+    fe = M.calcFwProj(A, x, "exact")
+    assert np.array_equal(fe, orc.fwproj(m.rowidx, m.elements, x))
+    f = M.calcFwProj(A, x, "fast")
+    u = M.calcBkProj(A, r, "fast")
+    assert rel_linf(f, orc.f64_fwproj(m.rowidx, m.elements, x)) < TOL_PROJ
+    assert rel_linf(u, orc.f64_bkproj(m.rowidx, m.elements, r, m.cols)) < TOL_PROJ
+    lhs = float(f.astype(np.float64) @ r.astype(np.float64))
+    rhs = float(x.astype(np.float64) @ u.astype(np.float64))
+    assert abs(lhs - rhs) <= 1e-6 * abs(lhs)
+
+
+def test_pet_fast_reconstruction(pet):
+    p, A = pet
+    m = p.matrix
+    K = 5
+    x, ll, tot = M.mlem(A, p.y, None, K, mode="fast", stats=True)
+    assert np.all(np.isfinite(x)) and np.all(x >= 0)
+    x64, ll64, _ = orc.f64_mlem(m.rowidx, m.elements, m.cols, p.y.astype(np.float64), K)
+    assert rel_linf(x, x64) < TOL_RECON
+    assert rel_linf(ll, ll64) < 1e-6
+    assert np.all(np.diff(ll) >= 0)
+    n = M.calcColumnSums(A).astype(np.float64)
+    ysum = float(p.y.astype(np.float64).sum())
+    assert abs(float(n @ x.astype(np.float64)) - ysum) <= 1e-5 * ysum
+    # tot[k] = sum_i f_i of iteration k = n . x_(k-1) = sum y for k >= 1
+    assert np.all(np.abs(tot[1:] - ysum) <= 1e-5 * ysum)
+
+
+def test_pet_batched_frames_bitwise(pet):
+    # two frames against the matrix, one RHS block: each frame equals the
+    # sequential fp32 reconstruction of its own sinogram
+    p, A = pet
+    m = p.matrix
+    Y = np.stack([p.y, np.maximum(p.y // 3, 0).astype(np.int32)])
+    budgets = np.array([3, 2], np.int32)
+    B = M.Batch(A, Y, budgets, block_width=4)
+    assert B.run(10) == 3
+    for f in range(2):
+        xo = orc.mlem(m.rowidx, m.elements, m.cols, Y[f], int(budgets[f]))[0]
+        assert np.array_equal(B.image(f), xo), f
+    B.close()
+
+
+@pytest.fixture(scope="module")
+def large():
+    p = synth.make_problem("large")
+    print(f"\n[fullsize] large: {p.matrix.rows} x {p.matrix.cols}, {p.matrix.nnz} nnz", flush=True)
+    A = _open(p)
+    yield p, A
+    A.close()
+
+
+def test_large_projections_and_counts(large):
+    p, A = large
+    m = p.matrix
+    rng = np.random.default_rng(12)
+    x = rng.uniform(0.1, 2.0, m.cols).astype(np.float32)
+    r = rng.uniform(0.1, 2.0, m.rows).astype(np.float32)
+    assert np.array_equal(M.calcFwProj(A, x, "exact"), orc.fwproj(m.rowidx, m.elements, x))
+    f = M.calcFwProj(A, x, "fast")
+    u = M.calcBkProj(A, r, "fast")
+    assert rel_linf(f, orc.f64_fwproj(m.rowidx, m.elements, x)) < TOL_PROJ
+    assert rel_linf(u, orc.f64_bkproj(m.rowidx, m.elements, r, m.cols)) < TOL_PROJ
+    lhs = float(f.astype(np.float64) @ r.astype(np.float64))
+    rhs = float(x.astype(np.float64) @ u.astype(np.float64))
+    assert abs(lhs - rhs) <= 1e-6 * abs(lhs)
+    xk = M.mlem(A, p.y, None, 2, mode="fast")
+    n = M.calcColumnSums(A).astype(np.float64)
+    ysum = float(p.y.astype(np.float64).sum())
+    assert abs(float(n @ xk.astype(np.float64)) - ysum) <= 1e-5 * ysum
