@@ -131,6 +131,40 @@ def cpu_baseline(problem, iters: int, parity_x: np.ndarray | None):
     return out, parity
 
 
+def streamed_run(A, prob, block_mib: int, iters: int, plain_value: float) -> dict:
+    """The same reconstruction with the matrix kept in pinned host memory and
+    streamed to the device in row blocks of at most `block_mib` MiB per part
+    (out-of-core path): iterations/s, host-to-device bytes per iteration
+    (every block's CSR part for the forward, CSC part for the backward) and the
+    PCIe rate they imply, plus the image's distance to the in-core run."""
+    import mlem_amd as M
+    m = prob.matrix
+    t0 = time.perf_counter()
+    S = M.Csr4Matrix.from_arrays(m.rows, m.cols, m.rowidx, m.elements,
+                                 stream_block_bytes=block_mib << 20)
+    open_s = time.perf_counter() - t0
+    nb = S.stream_blocks()
+    R = M.Recon(S, prob.y, mode="fast")
+    R.run(1)
+    M.device_sync()
+    t0 = time.perf_counter()
+    R.run(iters)
+    M.device_sync()
+    wall = time.perf_counter() - t0
+    xs = R.image()
+    R.close()
+    S.close()
+    xh = M.mlem(A, prob.y, None, iters + 1, mode="fast")
+    d = np.abs(xs.astype(np.float64) - xh.astype(np.float64))
+    floor = 1e-6 * float(np.max(np.abs(xh)))
+    moved = 16 * m.nnz + 4 * (m.rows + nb * (m.cols + 1))
+    return {"block_mib": block_mib, "blocks": nb, "iterations": iters,
+            "ms_per_iteration": 1e3 * wall / iters, "iterations_per_s": iters / wall,
+            "h2d_bytes_per_iteration": moved, "h2d_gbs": moved * iters / wall / 1e9,
+            "slowdown_vs_hbm": plain_value / (iters / wall), "open_s": open_s,
+            "rel_linf_vs_in_core": float(np.max(d / np.maximum(np.abs(xh), floor)))}
+
+
 def batched_run(A, prob, frames: int, width: int, plain_value: float) -> dict:
     """Dynamic-frame workload on the same matrix: per-frame phantom scaling
     U(0.2, 2) per ellipsoid, total counts log-uniform in [1e5, 1e8], iteration
@@ -203,6 +237,10 @@ def main() -> int:
     ap.add_argument("--unrolled-frames", type=int, default=8,
                     help="frames of the unrolled forward/backward timing (K = 20, "
                          "checkpoints every 5 iterations); 0 = skip")
+    ap.add_argument("--streamed-iters", type=int, default=5,
+                    help="iterations of the out-of-core (host-streamed) measurement; 0 = skip")
+    ap.add_argument("--streamed-block-mib", type=int, default=256,
+                    help="largest part of a streamed row block (MiB)")
     ap.add_argument("--fp64-iters", type=int, default=0,
                     help="also compare this many iterations with FP64 MLEM (oracle, all "
                          "host cores); 0 = skip")
@@ -303,6 +341,14 @@ def main() -> int:
             unrolled = {"error": f"{type(e).__name__}: {e}"}
         print(f"[bench] unrolled: {unrolled}", file=sys.stderr, flush=True)
 
+    streamed = None
+    if args.streamed_iters > 0 and args.precision == "fp32" and args.mode == "fast":
+        try:
+            streamed = streamed_run(A, prob, args.streamed_block_mib, args.streamed_iters, value)
+        except Exception as e:  # reported in the line, never hides the plain result
+            streamed = {"error": f"{type(e).__name__}: {e}"}
+        print(f"[bench] streamed: {streamed}", file=sys.stderr, flush=True)
+
     accuracy = None
     if args.fp64_iters > 0:
         accuracy = fp64_check(A, prob, args.precision, args.fp64_iters)
@@ -346,6 +392,7 @@ def main() -> int:
         "parity": parity,
         "batched": batched,
         "unrolled": unrolled,
+        "streamed": streamed,
         "fp64_check": accuracy,
     }
     print(json.dumps(line), flush=True)

@@ -126,6 +126,33 @@ MLEM_API int mlem_open_host(uint32_t rows, uint32_t cols, uint64_t nnz,
                    const uint64_t* rowidx, const void* elements,
                    const uint32_t grid[3], unsigned flags, mlem_matrix** out);
 
+/* Out-of-core matrices (larger than HBM). The elements stay in pinned host
+ * memory in blocks of whole rows; a block holds its rows (CSR) and their
+ * transpose (CSC), each part at most block_bytes bytes (0: a quarter of the
+ * device memory free at open). Every forward projection streams the CSR parts
+ * and every backprojection the CSC parts over PCIe through two device
+ * buffers, the copy of one block overlapping the kernels of the block before.
+ * A column's backprojection continues its running sum from block to block in
+ * ascending row order, so EXACT results equal the in-core ones bit for bit.
+ * Every step function, mode and mlem_reconstruct / mlem_recon_* work on such a
+ * handle (iterations run without hipGraph capture; statistics need at most 4096
+ * blocks); the launch configuration is fixed (lane groups), and the batched
+ * and unrolled paths, 16-bit values and the locality layout do not apply.
+ * Flags: MLEM_OPEN_EXPAND_SYMMETRY (file only); the others are ignored or
+ * rejected. Replaces the split-and-copy loop of
+ * mlem_cuda/versions/version_7.cu:205-260 with the row splitting of
+ * mlem_cuda/helper_files/helper_v_7_8.cu:4-72 (parts sized by free device
+ * memory). */
+MLEM_API int mlem_open_streamed(const char* path, unsigned flags, uint64_t block_bytes,
+                                mlem_matrix** out);
+MLEM_API int mlem_open_host_streamed(uint32_t rows, uint32_t cols, uint64_t nnz,
+                                     const uint64_t* rowidx, const void* elements,
+                                     const uint32_t grid[3], unsigned flags,
+                                     uint64_t block_bytes, mlem_matrix** out);
+/* Row blocks of a streamed matrix; 0 for a matrix held in HBM. No reference
+ * counterpart (version_7.cu prints its part count). */
+MLEM_API int mlem_stream_blocks(const mlem_matrix* m, uint32_t* blocks);
+
 /* Release device memory, stream and handle. Replaces Csr4Matrix::~Csr4Matrix
  * (csr4matrix.cpp:159-163). Accepts NULL. */
 MLEM_API void mlem_close(mlem_matrix* m);

@@ -89,11 +89,15 @@ class Csr4Matrix:
 
     precision "fp16" / "bf16" stores every row as an fp32 scale (the row's
     largest value) times 16-bit quotients; "fp32" keeps the file's values.
+    stream_block_bytes (not None) keeps the matrix in pinned host memory and
+    streams it to the device in row blocks of at most that many bytes per
+    part (0: sized from the free device memory), for matrices larger than HBM.
     """
 
     def __init__(self, path: Optional[str] = None, *, no_csc: bool = False,
                  keep_order: bool = False, precision: str = "fp32",
-                 expand_symmetry: bool = False, _handle=None):
+                 expand_symmetry: bool = False, stream_block_bytes: Optional[int] = None,
+                 _handle=None):
         self._h = None
         self.precision = precision
         if _handle is not None:
@@ -105,7 +109,11 @@ class Csr4Matrix:
             flags = _flags(no_csc, keep_order, precision)
             if expand_symmetry:
                 flags |= OPEN_EXPAND_SYMMETRY
-            check(lib().mlem_open(os.fsencode(path), flags, C.byref(h)))
+            if stream_block_bytes is None:
+                check(lib().mlem_open(os.fsencode(path), flags, C.byref(h)))
+            else:
+                check(lib().mlem_open_streamed(os.fsencode(path), flags,
+                                               int(stream_block_bytes), C.byref(h)))
             self._h = h
         r, c, n = C.c_uint32(), C.c_uint32(), C.c_uint64()
         check(lib().mlem_dims(self._h, C.byref(r), C.byref(c), C.byref(n)))
@@ -114,7 +122,8 @@ class Csr4Matrix:
     @classmethod
     def from_arrays(cls, rows: int, cols: int, rowidx: np.ndarray, elements: np.ndarray,
                     grid=None, no_csc: bool = False, keep_order: bool = False,
-                    precision: str = "fp32") -> "Csr4Matrix":
+                    precision: str = "fp32",
+                    stream_block_bytes: Optional[int] = None) -> "Csr4Matrix":
         """grid = (nx, ny, nz) enables the device locality layout."""
         flags = _flags(no_csc, keep_order, precision)
         rowidx = np.ascontiguousarray(rowidx, dtype=np.uint64)
@@ -126,9 +135,19 @@ class Csr4Matrix:
         nnz = int(elements.shape[0])
         g = None if grid is None else (C.c_uint32 * 3)(*grid)
         h = C.c_void_p()
-        check(lib().mlem_open_host(rows, cols, nnz, _ptr(rowidx), _ptr(elements), g,
-                                   flags, C.byref(h)))
+        if stream_block_bytes is None:
+            check(lib().mlem_open_host(rows, cols, nnz, _ptr(rowidx), _ptr(elements), g,
+                                       flags, C.byref(h)))
+        else:
+            check(lib().mlem_open_host_streamed(rows, cols, nnz, _ptr(rowidx), _ptr(elements),
+                                                g, flags, int(stream_block_bytes), C.byref(h)))
         return cls(_handle=h, precision=precision)
+
+    def stream_blocks(self) -> int:
+        """Row blocks of a streamed matrix (0 when the matrix is in HBM)."""
+        n = C.c_uint32()
+        check(lib().mlem_stream_blocks(self._h, C.byref(n)))
+        return n.value
 
     def rows(self) -> int:
         return self._rows

@@ -150,6 +150,8 @@ mlem_matrix::~mlem_matrix() {
     dfree(m.tiles_bwd);
     free_dict(m.dfwd);
     free_dict(m.dbwd);
+    delete m.streamed;
+    m.streamed = nullptr;
     if (m.stream) (void)hipStreamDestroy(m.stream);
   }
 }
@@ -218,8 +220,7 @@ float time_variant(hipStream_t s, Launch&& launch, int reps) {
   return t[t.size() / 2];
 }
 
-// Measure the forward and backward kernels for every lane count and keep the
-// fastest. Scratch vectors are filled with 1 so the timing sees real data.
+// Launch configuration of a kernel family and its parameter (validated).
 GroupLaunch make_launch(int kind, int param) {
   GroupLaunch g;
   g.kind = kind;
@@ -437,6 +438,11 @@ void d2h(void* h, const void* d, size_t n, hipStream_t s) {
 
 constexpr unsigned kStatsRing = 4096;
 
+// Forward launches per projection: one, or one per block of a streamed matrix.
+unsigned fwd_launches(const DevMatrix& m) {
+  return m.streamed ? (unsigned)std::max<size_t>(m.streamed->blocks.size(), 1) : 1u;
+}
+
 void enqueue_iteration(mlem_recon* R) {
   DevMatrix& m = R->mat->m;
   const StatsState* st = R->want_stats ? &R->st : nullptr;
@@ -450,15 +456,23 @@ void enqueue_iteration(mlem_recon* R) {
   }
 }
 
+// Iteration k's statistics: the per-launch results of its forward launches
+// (one, or one per streamed block), summed in launch order.
 void collect_stats(mlem_recon* R, int first, int count) {
   if (!R->want_stats || count <= 0) return;
+  const unsigned per = fwd_launches(R->mat->m);
   std::vector<double2> res(kStatsRing);
   d2h(res.data(), R->st.results, sizeof(double2) * kStatsRing, R->mat->m.stream);
   MLEM_HIP(hipStreamSynchronize(R->mat->m.stream));
   for (int k = 0; k < count; ++k) {
-    const double2 v = res[(unsigned)(first + k) % kStatsRing];
-    R->loglik.push_back(v.x);
-    R->fwdtot.push_back(v.y);
+    double ll = 0.0, tot = 0.0;
+    for (unsigned b = 0; b < per; ++b) {
+      const double2 v = res[((uint64_t)(first + k) * per + b) % kStatsRing];
+      ll += v.x;
+      tot += v.y;
+    }
+    R->loglik.push_back(ll);
+    R->fwdtot.push_back(tot);
   }
 }
 
@@ -466,7 +480,8 @@ void run_iterations(mlem_recon* R, int iterations) {
   DevMatrix& m = R->mat->m;
   int left = iterations;
   while (left > 0) {
-    const int n = R->want_stats ? std::min<int>(left, (int)kStatsRing) : left;
+    const int n =
+        R->want_stats ? std::min<int>(left, (int)(kStatsRing / fwd_launches(m))) : left;
     if (R->use_graph) {
       if (!R->exec) {
         MLEM_HIP(hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal));
@@ -502,14 +517,18 @@ mlem_recon* create_recon(mlem_matrix* h, const int32_t* y, uint32_t rows, const 
   if (!o) o = &dflt;
   if (o->mode < MLEM_MODE_FAST || o->mode > MLEM_MODE_ATOMIC) raise(MLEM_EINVAL, "unknown mode");
   if (o->init == MLEM_INIT_GIVEN && !x0) raise(MLEM_EINVAL, "MLEM_INIT_GIVEN needs an image");
-  if (o->mode != MLEM_MODE_ATOMIC && !h->m.csc)
+  if (o->mode != MLEM_MODE_ATOMIC && !h->m.csc && !h->m.streamed)
     raise(MLEM_ESTATE, "matrix was opened without a CSC; only MLEM_MODE_ATOMIC is available");
   DevMatrix& m = h->m;
   std::unique_ptr<mlem_recon> R(new mlem_recon());
   R->mat = h;
   R->mode = o->mode;
-  R->use_graph = o->use_graph != 0;
+  // a streamed iteration spans two streams and host copies: run it directly
+  R->use_graph = o->use_graph != 0 && !m.streamed;
   R->want_stats = (o->loglik != nullptr) || (o->fwd_total != nullptr);
+  if (R->want_stats && fwd_launches(m) > kStatsRing)
+    raise(MLEM_EINVAL, "statistics need at most " + std::to_string(kStatsRing) +
+                           " streamed blocks");
   MLEM_HIP(hipMalloc(&R->y, 4ull * std::max<uint32_t>(rows, 1)));
   MLEM_HIP(hipMalloc(&R->x, 4ull * std::max<uint32_t>(cols, 1)));
   MLEM_HIP(hipMalloc(&R->r, 4ull * std::max<uint32_t>(rows, 1)));
@@ -694,6 +713,133 @@ int mlem_open_host(uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t* r
     h->load_seconds = seconds_since(t0);
     finish_open(h.get(), flags);
     *out = h.release();
+  });
+}
+
+namespace {
+
+// Handle of a streamed matrix: no device CSR/CSC, the row blocks in pinned
+// host memory; column sums by a streamed EXACT backprojection of ones, which
+// adds every column's values in ascending row order (calcColumnSums,
+// seqcsr4mlem.cpp:37-49).
+mlem_matrix* open_streamed(uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t* rowidx,
+                           const ElementFetch& fetch, const uint32_t grid[3], unsigned flags,
+                           uint64_t block_bytes) {
+  if (flags & (MLEM_OPEN_FP16 | MLEM_OPEN_BF16))
+    raise(MLEM_EINVAL, "streamed matrices hold fp32 values");
+  if (rows == 0xFFFFFFFFu || cols == 0xFFFFFFFFu) raise(MLEM_EINVAL, "dimension too large");
+  std::unique_ptr<mlem_matrix> h(new mlem_matrix());
+  DevMatrix& m = h->m;
+  m.rows = rows;
+  m.cols = cols;
+  m.nnz = nnz;
+  if (grid)
+    for (int k = 0; k < 3; ++k) m.grid[k] = grid[k];
+  select_device(&m.device, &m.cus);
+  MLEM_HIP(hipStreamCreateWithFlags(&m.stream, hipStreamNonBlocking));
+  auto t0 = std::chrono::steady_clock::now();
+  m.streamed = build_streamed(rows, cols, nnz, rowidx, fetch, block_bytes, m.stream);
+  h->load_seconds = seconds_since(t0);
+  t0 = std::chrono::steady_clock::now();
+  MLEM_HIP(hipMalloc(&m.norm, 4ull * std::max<uint32_t>(cols, 1)));
+  ensure_scratch(m);
+  launch_fill(m.norm, 0.f, cols, m.stream);
+  launch_fill(m.sr, 1.f, rows, m.stream);
+  stream_bwd_add(m, true, m.sr, m.norm, m.stream);
+  std::vector<float> norm(cols);
+  d2h(norm.data(), m.norm, 4ull * cols, m.stream);
+  MLEM_HIP(hipStreamSynchronize(m.stream));
+  double sn = 0.0;  // initImage's double sum, index order (seqcsr4mlem.cpp:55-56)
+  for (uint32_t j = 0; j < cols; ++j) sn += norm[j];
+  m.sumnorm = sn;
+  h->norm_seconds = seconds_since(t0);
+  m.fwd = make_launch(kGroups, m.streamed->lanes_fwd);
+  m.bwd = make_launch(kGroups, m.streamed->lanes_bwd);
+  m.scatter = m.bwd;
+  return h.release();
+}
+
+}  // namespace
+
+int mlem_open_streamed(const char* path, unsigned flags, uint64_t block_bytes,
+                       mlem_matrix** out) {
+  return guarded([&] {
+    if (!path || !out) raise(MLEM_EINVAL, "null argument");
+    *out = nullptr;
+    Csr4Header hd;
+    std::vector<uint64_t> rowidx;
+    int fd = -1;
+    csr4_read_header(path, hd, rowidx, &fd);
+    struct FdClose {
+      int fd;
+      ~FdClose() {
+        if (fd >= 0) close(fd);
+      }
+    } fdc{fd};
+    const std::string fn(path);
+    std::unique_ptr<mlem_matrix> h;
+    if ((flags & MLEM_OPEN_EXPAND_SYMMETRY) && (hd.flags & 7u)) {
+      std::vector<Csr4Element> el(hd.nnz);
+      char* p = reinterpret_cast<char*>(el.data());
+      for (uint64_t done = 0, n = 8ull * hd.nnz; done < n;) {
+        ssize_t got = pread(fd, p + done, n - done, (off_t)(hd.dataOffset + done));
+        if (got <= 0) raise(MLEM_EIO, "Cannot read matrix elements from " + fn);
+        done += (uint64_t)got;
+      }
+      std::vector<uint64_t> xrow;
+      std::vector<Csr4Element> xel;
+      expand_symmetry(hd.scanner, hd.flags & 7u, hd.nRows, hd.nColumns, rowidx, el.data(), xrow,
+                      xel);
+      el.clear();
+      el.shrink_to_fit();
+      const char* src = reinterpret_cast<const char*>(xel.data());
+      h.reset(open_streamed(
+          hd.nRows, hd.nColumns, xel.size(), xrow.data(),
+          [&](void* dst, uint64_t first, uint64_t count) {
+            std::memcpy(dst, src + 8 * first, 8 * count);
+          },
+          hd.scanner.grid, flags, block_bytes));
+    } else {
+      h.reset(open_streamed(
+          hd.nRows, hd.nColumns, hd.nnz, rowidx.data(),
+          [&](void* dst, uint64_t first, uint64_t count) {
+            char* p = static_cast<char*>(dst);
+            const uint64_t n = 8 * count;
+            for (uint64_t done = 0; done < n;) {
+              ssize_t got = pread(fd, p + done, n - done,
+                                  (off_t)(hd.dataOffset + 8 * first + done));
+              if (got <= 0) raise(MLEM_EIO, "Cannot read matrix elements from " + fn);
+              done += (uint64_t)got;
+            }
+          },
+          hd.scanner.grid, flags, block_bytes));
+      h->m.symflags = hd.flags & 7u;
+    }
+    *out = h.release();
+  });
+}
+
+int mlem_open_host_streamed(uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t* rowidx,
+                            const void* elements, const uint32_t grid[3], unsigned flags,
+                            uint64_t block_bytes, mlem_matrix** out) {
+  return guarded([&] {
+    if (!out || (rows && !rowidx) || (nnz && !elements)) raise(MLEM_EINVAL, "null argument");
+    *out = nullptr;
+    csr4_check_rowidx(rowidx, rows, nnz);
+    const char* src = static_cast<const char*>(elements);
+    *out = open_streamed(
+        rows, cols, nnz, rowidx,
+        [&](void* dst, uint64_t first, uint64_t count) {
+          std::memcpy(dst, src + 8 * first, 8 * count);
+        },
+        grid, flags, block_bytes);
+  });
+}
+
+int mlem_stream_blocks(const mlem_matrix* h, uint32_t* blocks) {
+  return guarded([&] {
+    if (!h || !blocks) raise(MLEM_EINVAL, "null argument");
+    *blocks = h->m.streamed ? (uint32_t)h->m.streamed->blocks.size() : 0u;
   });
 }
 
@@ -1012,6 +1158,7 @@ int mlem_get_launch(const mlem_matrix* h, int* fk, int* fp, int* bk, int* bp) {
 int mlem_set_launch(mlem_matrix* h, int fk, int fp, int bk, int bp) {
   return guarded([&] {
     if (!h) raise(MLEM_EINVAL, "null matrix handle");
+    if (h->m.streamed) raise(MLEM_EINVAL, "streamed matrices have a fixed launch configuration");
     const GroupLaunch f = make_launch(fk, fp), b = make_launch(bk, bp);
     h->m.fwd = f;
     h->m.bwd = b;
@@ -1022,6 +1169,7 @@ int mlem_set_launch(mlem_matrix* h, int fk, int fp, int bk, int bp) {
 int mlem_autotune(mlem_matrix* h) {
   return guarded([&] {
     if (!h) raise(MLEM_EINVAL, "null matrix handle");
+    if (h->m.streamed) raise(MLEM_EINVAL, "streamed matrices have a fixed launch configuration");
     autotune(h->m);
   });
 }

@@ -461,6 +461,7 @@ int mlem_batch_create(mlem_matrix* h, uint32_t frames, const int32_t* y, uint32_
     *out = nullptr;
     DevMatrix& m = h->m;
     if (rows != m.rows) raise(MLEM_EINVAL, "sinogram length does not match the matrix rows");
+    if (m.streamed) raise(MLEM_EINVAL, "batched reconstruction needs a matrix held in HBM");
     if (!m.csc) raise(MLEM_ESTATE, "batched reconstruction needs the CSC transpose");
     if (m.vk != kF32) raise(MLEM_EINVAL, "batched reconstruction needs fp32 matrix values");
     mlem_batch_options o;

@@ -71,6 +71,20 @@ int main(int argc, char* argv[]) {
       return fail(std::string("unknown MLEM_SYMMETRY ") + s);
   }
 
+  // MLEM_STREAM=<MiB>: keep the matrix in host memory and stream it in row
+  // blocks of at most that many MiB per part (0: sized from free device memory)
+  bool streamed = false;
+  uint64_t block_bytes = 0;
+  if (const char* s = std::getenv("MLEM_STREAM")) {
+    if (*s) {
+      int mib = 0;
+      if (parse_int(s, "MLEM_STREAM block size (MiB)", &mib)) return 1;
+      if (mib < 0) return fail("MLEM_STREAM must be >= 0");
+      streamed = true;
+      block_bytes = (uint64_t)mib << 20;
+    }
+  }
+
   std::cout << "Matrix file: " << mtx << std::endl;
   std::cout << "Input file: " << in << std::endl;
   std::cout << "Output file: " << out << std::endl;
@@ -85,8 +99,10 @@ int main(int argc, char* argv[]) {
   }
 
   mlem_matrix* m = nullptr;
-  if (mlem_open(mtx.c_str(), open_flags, &m) != MLEM_OK)
-    return fail(mlem_last_error());
+  const int orc = streamed ? mlem_open_streamed(mtx.c_str(), open_flags & MLEM_OPEN_EXPAND_SYMMETRY,
+                                                 block_bytes, &m)
+                           : mlem_open(mtx.c_str(), open_flags, &m);
+  if (orc != MLEM_OK) return fail(mlem_last_error());
   uint32_t rows = 0, cols = 0;
   uint64_t nnz = 0;
   mlem_dims(m, &rows, &cols, &nnz);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -87,6 +88,34 @@ struct GroupLaunch {
   int blocks_per_cu = 0;   // kGroups: 0 = one block per 256/lanes rows
 };
 
+// A matrix kept in pinned host memory and streamed to the device in row
+// blocks (out-of-core). Each block holds its rows as CSR (block-local row
+// pointers, global columns) and their transpose as CSC (global rows); a
+// projection copies its part of every block in turn through two device
+// buffers, the copy of the next block overlapping the kernels of this one.
+struct StreamBlock {
+  uint32_t row0 = 0, rows = 0;  // first row, rows
+  uint64_t nnz = 0;
+  uint64_t csr_off = 0, csr_bytes = 0;  // parts within the host buffer
+  uint64_t csc_off = 0, csc_bytes = 0;
+};
+struct StreamedMatrix {
+  char* host = nullptr;  // pinned: every block's CSR part, then its CSC part
+  uint64_t host_bytes = 0;
+  std::vector<StreamBlock> blocks;
+  uint64_t part_bytes = 0;  // largest part: the size of each device buffer
+  char* dev[2] = {nullptr, nullptr};
+  hipStream_t copy = nullptr;
+  hipEvent_t copied[2] = {nullptr, nullptr};  // part in dev[i]
+  hipEvent_t freed[2] = {nullptr, nullptr};   // kernels reading dev[i] done
+  float* u = nullptr;                          // [cols] backprojection sums
+  int lanes_fwd = 16, lanes_bwd = 4;           // FAST lane groups per row / column
+  StreamedMatrix() = default;
+  StreamedMatrix(const StreamedMatrix&) = delete;
+  StreamedMatrix& operator=(const StreamedMatrix&) = delete;
+  ~StreamedMatrix();
+};
+
 struct DevMatrix {
   uint32_t rows = 0, cols = 0;
   uint64_t nnz = 0;
@@ -137,6 +166,8 @@ struct DevMatrix {
   int32_t* sy = nullptr; // [rows]
   float* tx = nullptr;   // [cols] file-order staging
   float* tf = nullptr;   // [rows] file-order staging
+
+  StreamedMatrix* streamed = nullptr;  // out-of-core matrix (no device CSR/CSC)
 };
 
 // Deterministic FP64 reduction state for the forward pass: per-block partial
@@ -168,6 +199,23 @@ void launch_bwd(const DevMatrix& m, int mode, const float* r, const float* norm,
                 float* out, bool update, hipStream_t s);
 // Backprojection by atomic scatter over the CSR: u += A^T r (u pre-zeroed).
 void launch_scatter(const DevMatrix& m, const float* r, float* u, hipStream_t s);
+// One row block of a streamed matrix (kernels.hip).
+void launch_fwd_block(const uint32_t* rowptr, const uint2* csr, uint32_t rows, int lanes,
+                      bool exact, const float* x, const int32_t* y, float* out, bool ratio,
+                      const StatsState* st, int cus, hipStream_t s);
+void launch_bwd_block_add(const uint32_t* colptr, const uint2* csc, uint32_t cols, int lanes,
+                          bool exact, const float* r, float* u, int cus, hipStream_t s);
+// Streamed matrices (stream.cpp): build the blocks from a row index (END
+// offsets) and an element source; the projections over every block.
+using ElementFetch = std::function<void(void* dst, uint64_t first, uint64_t count)>;
+StreamedMatrix* build_streamed(uint32_t rows, uint32_t cols, uint64_t nnz,
+                               const uint64_t* rowidx, const ElementFetch& fetch,
+                               uint64_t block_bytes, hipStream_t s);
+void stream_fwd(const DevMatrix& m, int mode, const float* x, const int32_t* y, float* out,
+                bool ratio, const StatsState* st, hipStream_t s);
+void stream_bwd(const DevMatrix& m, int mode, const float* r, const float* norm, float* out,
+                bool update, hipStream_t s);
+void stream_bwd_add(const DevMatrix& m, bool exact, const float* r, float* u, hipStream_t s);
 // x *= (n != 0) ? u/n : u; optionally u = 0 afterwards.
 void launch_update(uint32_t cols, const float* u, const float* norm, float* x,
                    bool zero_u, hipStream_t s);

@@ -69,16 +69,18 @@ __device__ __forceinline__ float group_sum(float v) {
   }
 }
 
-// Dot product of one sparse row (or CSC column) with a dense vector.
+// Dot product of one sparse row (or CSC column) with a dense vector. EXACT
+// continues a running sum that starts at `init` (0 for a whole segment).
 template <int G, bool EXACT>
 __device__ __forceinline__ float seg_dot(const uint2* __restrict__ a, const float* __restrict__ v,
-                                         uint32_t beg, uint32_t end, uint32_t lane) {
+                                         uint32_t beg, uint32_t end, uint32_t lane,
+                                         float init = 0.f) {
   if constexpr (EXACT) {
     // one lane, ascending element order; the records and gathers of 8
     // elements are issued together (clamped at the segment end), the products
     // then added one by one
     constexpr int U = 8;
-    float acc = 0.f;
+    float acc = init;
     for (uint32_t k = beg; k < end; k += U) {
       u32x2 e[U];
       float g[U];
@@ -255,7 +257,13 @@ __global__ __launch_bounds__(kBlock) void k_fwd(
 }
 
 // --------------------------------------------------------------- backward --
-template <int G, bool EXACT, bool UPDATE>
+// OUT: what a column's sum u becomes. kBwdSet: out = u; kBwdUpdate:
+// out *= (n != 0) ? u/n : u; kBwdAdd: out += u, the sum of one row block's
+// part of the column (EXACT continues out's running sum element by element,
+// so adding the blocks in ascending row order reproduces the whole column's
+// sequential sum).
+constexpr int kBwdSet = 0, kBwdUpdate = 1, kBwdAdd = 2;
+template <int G, bool EXACT, int OUT>
 __global__ __launch_bounds__(kBlock) void k_bwd(const uint32_t* __restrict__ colptr,
                                                 const uint2* __restrict__ csc,
                                                 const float* __restrict__ r,
@@ -271,12 +279,17 @@ __global__ __launch_bounds__(kBlock) void k_bwd(const uint32_t* __restrict__ col
     const uint32_t col = (uint32_t)(base + grp);
     const bool active = base + grp < c1;
     float acc = 0.f;
-    if (active) acc = seg_dot<G, EXACT>(csc, r, colptr[col], colptr[col + 1], lane);
+    if (active) {
+      const float init = (EXACT && OUT == kBwdAdd) ? out[col] : 0.f;
+      acc = seg_dot<G, EXACT>(csc, r, colptr[col], colptr[col + 1], lane, init);
+    }
     acc = group_sum<G>(acc);
     if (active && lane == 0) {
-      if constexpr (UPDATE) {
+      if constexpr (OUT == kBwdUpdate) {
         const float n = norm[col];
         out[col] = out[col] * ((n != 0.f) ? acc / n : acc);
+      } else if constexpr (OUT == kBwdAdd && !EXACT) {
+        out[col] = out[col] + acc;
       } else {
         out[col] = acc;
       }
@@ -1049,11 +1062,42 @@ void bwd_g(const DevMatrix& m, const float* r, const float* norm, float* out, bo
   const Geometry g = geometry(m.cols, G, m.bwd.blocks_per_cu, m.cus, false);
   const dim3 grid(g.grid), block(kBlock);
   if (update)
-    hipLaunchKernelGGL((k_bwd<G, EXACT, true>), grid, block, 0, s, m.colptr, m.csc, r, norm, out,
-                       m.cols, g.chunk);
+    hipLaunchKernelGGL((k_bwd<G, EXACT, kBwdUpdate>), grid, block, 0, s, m.colptr, m.csc, r, norm,
+                       out, m.cols, g.chunk);
   else
-    hipLaunchKernelGGL((k_bwd<G, EXACT, false>), grid, block, 0, s, m.colptr, m.csc, r, norm, out,
-                       m.cols, g.chunk);
+    hipLaunchKernelGGL((k_bwd<G, EXACT, kBwdSet>), grid, block, 0, s, m.colptr, m.csc, r, norm,
+                       out, m.cols, g.chunk);
+}
+
+// One row block of a streamed matrix: its CSR (block-local row pointers,
+// global columns; y and out start at the block's first row) and its CSC
+// (global rows), one lane group per row or column.
+template <int G, bool EXACT>
+void fwd_block_g(const uint32_t* rowptr, const uint2* csr, uint32_t rows, const float* x,
+                 const int32_t* y, float* out, bool ratio, const StatsState* st, int cus,
+                 hipStream_t s) {
+  const bool stats = ratio && st != nullptr;
+  const Geometry g = geometry(rows, G, 0, cus, stats);
+  if (stats && g.grid > st->max_blocks) raise(MLEM_ESTATE, "statistics buffer too small for launch");
+  const dim3 grid(g.grid), block(kBlock);
+  if (stats)
+    hipLaunchKernelGGL((k_fwd<G, EXACT, true, true>), grid, block, 0, s, rowptr, csr, x, y, out,
+                       (const float*)nullptr, rows, g.chunk, st->partials, st->counter,
+                       st->results, st->ring);
+  else if (ratio)
+    hipLaunchKernelGGL((k_fwd<G, EXACT, true, false>), grid, block, 0, s, rowptr, csr, x, y, out,
+                       (const float*)nullptr, rows, g.chunk, nullptr, nullptr, nullptr, 0u);
+  else
+    hipLaunchKernelGGL((k_fwd<G, EXACT, false, false>), grid, block, 0, s, rowptr, csr, x, y, out,
+                       (const float*)nullptr, rows, g.chunk, nullptr, nullptr, nullptr, 0u);
+}
+
+template <int G, bool EXACT>
+void bwd_block_add_g(const uint32_t* colptr, const uint2* csc, uint32_t cols, const float* r,
+                     float* u, int cus, hipStream_t s) {
+  const Geometry g = geometry(cols, G, 0, cus, false);
+  hipLaunchKernelGGL((k_bwd<G, EXACT, kBwdAdd>), dim3(g.grid), dim3(kBlock), 0, s, colptr, csc, r,
+                     (const float*)nullptr, u, cols, g.chunk);
 }
 
 template <int G>
@@ -1361,6 +1405,10 @@ unsigned stats_max_blocks(const DevMatrix& m) {
 
 void launch_fwd(const DevMatrix& m, int mode, const float* x, const int32_t* y, float* out,
                 bool ratio, const StatsState* stats, hipStream_t s) {
+  if (m.streamed) {
+    stream_fwd(m, mode, x, y, out, ratio, stats, s);
+    return;
+  }
   if (mode == MLEM_MODE_EXACT) {
     fwd_g<1, true>(m, x, y, out, ratio, stats, s);
   } else if (is_dict(m.fwd.kind)) {
@@ -1398,6 +1446,10 @@ void launch_fwd(const DevMatrix& m, int mode, const float* x, const int32_t* y, 
 
 void launch_bwd(const DevMatrix& m, int mode, const float* r, const float* norm, float* out,
                 bool update, hipStream_t s) {
+  if (m.streamed) {
+    stream_bwd(m, mode, r, norm, out, update, s);
+    return;
+  }
   if (!m.csc) raise(MLEM_ESTATE, "matrix was opened without a CSC transpose");
   if (mode == MLEM_MODE_EXACT) {
     bwd_g<1, true>(m, r, norm, out, update, s);
@@ -1434,7 +1486,46 @@ void launch_bwd(const DevMatrix& m, int mode, const float* r, const float* norm,
   check_launch("backprojection launch");
 }
 
+void launch_fwd_block(const uint32_t* rowptr, const uint2* csr, uint32_t rows, int lanes,
+                      bool exact, const float* x, const int32_t* y, float* out, bool ratio,
+                      const StatsState* st, int cus, hipStream_t s) {
+  if (exact) {
+    fwd_block_g<1, true>(rowptr, csr, rows, x, y, out, ratio, st, cus, s);
+  } else {
+    switch (lanes) {
+      case 4: fwd_block_g<4, false>(rowptr, csr, rows, x, y, out, ratio, st, cus, s); break;
+      case 8: fwd_block_g<8, false>(rowptr, csr, rows, x, y, out, ratio, st, cus, s); break;
+      case 16: fwd_block_g<16, false>(rowptr, csr, rows, x, y, out, ratio, st, cus, s); break;
+      case 32: fwd_block_g<32, false>(rowptr, csr, rows, x, y, out, ratio, st, cus, s); break;
+      case 64: fwd_block_g<64, false>(rowptr, csr, rows, x, y, out, ratio, st, cus, s); break;
+      default: raise(MLEM_EINVAL, "unsupported forward lane count");
+    }
+  }
+  check_launch("streamed forward projection launch");
+}
+
+void launch_bwd_block_add(const uint32_t* colptr, const uint2* csc, uint32_t cols, int lanes,
+                          bool exact, const float* r, float* u, int cus, hipStream_t s) {
+  if (exact) {
+    bwd_block_add_g<1, true>(colptr, csc, cols, r, u, cus, s);
+  } else {
+    switch (lanes) {
+      case 4: bwd_block_add_g<4, false>(colptr, csc, cols, r, u, cus, s); break;
+      case 8: bwd_block_add_g<8, false>(colptr, csc, cols, r, u, cus, s); break;
+      case 16: bwd_block_add_g<16, false>(colptr, csc, cols, r, u, cus, s); break;
+      case 32: bwd_block_add_g<32, false>(colptr, csc, cols, r, u, cus, s); break;
+      case 64: bwd_block_add_g<64, false>(colptr, csc, cols, r, u, cus, s); break;
+      default: raise(MLEM_EINVAL, "unsupported backward lane count");
+    }
+  }
+  check_launch("streamed backprojection launch");
+}
+
 void launch_scatter(const DevMatrix& m, const float* r, float* u, hipStream_t s) {
+  if (m.streamed) {  // u += A^T r, one block after another (no atomics needed)
+    stream_bwd_add(m, false, r, u, s);
+    return;
+  }
   switch (m.scatter.lanes) {
     case 4: scatter_g<4>(m, r, u, s); break;
     case 8: scatter_g<8>(m, r, u, s); break;

@@ -319,6 +319,7 @@ int mlem_forward_k(mlem_matrix* h, uint32_t frames, const float* y, uint32_t row
       raise(MLEM_EINVAL, "vector lengths do not match the matrix dimensions");
     if (K < 1) raise(MLEM_EINVAL, "K must be at least 1");
     if (checkpoint_interval < 0) raise(MLEM_EINVAL, "negative checkpoint interval");
+    if (m.streamed) raise(MLEM_EINVAL, "unrolled MLEM needs a matrix held in HBM");
     if (!m.csc) raise(MLEM_ESTATE, "unrolled MLEM needs the CSC transpose");
     if (m.vk != kF32) raise(MLEM_EINVAL, "unrolled MLEM needs fp32 matrix values");
     std::unique_ptr<mlem_tape> T(new mlem_tape());
