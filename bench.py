@@ -104,7 +104,25 @@ def fp64_check(A, prob, precision: str, iters: int) -> dict:
     return out
 
 
-def cpu_baseline(problem, iters: int, parity_x: np.ndarray | None):
+def parity_check(A, prob, iters: int) -> dict:
+    """The image after `iters` iterations in FAST and in EXACT mode against the
+    same iterations of the fp32 sequential restatement of seqcsr4mlem."""
+    import mlem_amd as M
+    from oracle import oracle as orc  # parity check only
+    m = prob.matrix
+    xo = orc.mlem(m.rowidx, m.elements, m.cols, prob.y, iters)[0]
+    floor = 1e-6 * float(np.max(np.abs(xo)))
+    out = {"iterations": iters}
+    for mode in ("fast", "exact"):
+        x = M.mlem(A, prob.y, None, iters, mode=mode)
+        d = np.abs(x.astype(np.float64) - xo.astype(np.float64))
+        out[mode] = {"rel_linf_vs_seq_fp32": float(np.max(d / np.maximum(np.abs(xo), floor))),
+                     "rel_l2_vs_seq_fp32": float(np.linalg.norm(d) / np.linalg.norm(xo)),
+                     "bit_identical": bool(np.array_equal(x, xo))}
+    return out
+
+
+def cpu_baseline(problem, iters: int):
     from oracle import oracle as orc  # CPU baseline leg only
     m = problem.matrix
     x, _, _, _, secs = orc.mlem(m.rowidx, m.elements, m.cols, problem.y, iters)
@@ -121,14 +139,7 @@ def cpu_baseline(problem, iters: int, parity_x: np.ndarray | None):
     out["openmp"] = {"value": 1.0 / omed, "unit": "iterations/s", "cores": threads,
                      "sample": f"{iters} iterations of the OpenMP restatement, median "
                                f"{omed:.3f} s per iteration"}
-    parity = None
-    if parity_x is not None:
-        d = np.abs(parity_x.astype(np.float64) - x.astype(np.float64))
-        floor = 1e-6 * float(np.max(np.abs(x)))
-        parity = {"iterations": iters,
-                  "rel_linf_vs_seq_fp32": float(np.max(d / np.maximum(np.abs(x), floor))),
-                  "rel_l2_vs_seq_fp32": float(np.linalg.norm(d) / np.linalg.norm(x))}
-    return out, parity
+    return out
 
 
 def streamed_run(A, prob, block_mib: int, iters: int, plain_value: float) -> dict:
@@ -221,6 +232,9 @@ def main() -> int:
     ap.add_argument("--mode", default="fast", choices=["fast", "atomic", "exact"])
     ap.add_argument("--cpu-iters", type=int, default=4,
                     help="iterations of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--parity-iters", type=int, default=60,
+                    help="iterations of the FAST / EXACT image checked against the fp32 "
+                         "sequential restatement (0 = skip)")
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--keep-order", action="store_true",
                     help="keep file row/voxel order on the device (no locality layout)")
@@ -314,16 +328,14 @@ def main() -> int:
         if t.get("launch") == json.loads(json.dumps(launch)):
             traffic = t.get(dom)
 
-    # parity sample + CPU baseline (same iteration count)
-    parity = None
     cpu = None
     if args.cpu_iters > 0:
-        R2 = M.Recon(A, prob.y, mode=args.mode)
-        R2.run(args.cpu_iters)
-        xg = R2.image()
-        R2.close()
-        cpu, parity = cpu_baseline(prob, args.cpu_iters, xg)
+        cpu = cpu_baseline(prob, args.cpu_iters)
         cpu["threads_available"] = os.cpu_count()
+    parity = None
+    if args.parity_iters > 0 and args.precision == "fp32":
+        parity = parity_check(A, prob, args.parity_iters)
+        print(f"[bench] parity: {parity}", file=sys.stderr, flush=True)
 
     batched = None
     if args.batched_frames > 0 and args.mode != "atomic":

@@ -45,6 +45,40 @@ def test_pet_exact_iterates_bitwise(pet):
     assert np.array_equal(x, orc.mlem(m.rowidx, m.elements, m.cols, p.y, 2)[0])
 
 
+def test_pet_60_iterations_vs_seq_fp32(pet):
+    # the benchmark's 60 iterations: EXACT bit-identical to the fp32
+    # restatement of seqcsr4mlem, FAST within 1e-5 relative per voxel
+    p, A = pet
+    m = p.matrix
+    xo = orc.mlem(m.rowidx, m.elements, m.cols, p.y, 60)[0]
+    assert np.array_equal(M.mlem(A, p.y, None, 60, mode="exact"), xo)
+    assert rel_linf(M.mlem(A, p.y, None, 60, mode="fast"), xo) < 1e-5
+
+
+def test_pet_unrolled_gradient_vs_fp64(pet):
+    # K = 20 unrolled iterations over a batch of 8 relaxed-count sinograms,
+    # checkpoints every 5; loss 0.5 |x_K - x_true|^2; frame 0's gradients
+    # against the FP64 reverse mode of the oracle
+    p, A = pet
+    m = p.matrix
+    rng = np.random.default_rng(31)
+    Y = np.stack([p.y.astype(np.float32)] +
+                 [(p.ybar * rng.uniform(0.3, 2.0)).astype(np.float32) for _ in range(7)])
+    u = M.Unrolled(A, Y, None, K=20, checkpoint_interval=5)
+    g = (u.xK - p.x_true[None, :]).astype(np.float32)
+    gy, gx0 = u.backward(g)
+    xK = u.xK[0].copy()
+    u.close()
+    y0 = Y[0]
+    x0 = np.full(m.cols, np.float32(np.sum(y0.astype(np.float64)) /
+                                    np.sum(M.calcColumnSums(A).astype(np.float64))))
+    xK64, gy64, gx064 = orc.f64_unroll(m.rowidx, m.elements, m.cols, y0, x0, 20,
+                                       gK=g[0].astype(np.float64))
+    assert rel_linf(xK, xK64) < 1e-4
+    for gg, g64 in ((gy[0], gy64), (gx0[0], gx064)):
+        assert np.linalg.norm(gg.astype(np.float64) - g64) <= 1e-4 * np.linalg.norm(g64)
+
+
 def test_pet_projections(pet):
     p, A = pet
     m = p.matrix

@@ -137,7 +137,7 @@ def test_finite_differences_and_homogeneity(tiny_u):
         xK = orc.f64_unroll(m.rowidx, m.elements, m.cols, yy, xx, K)
         return 0.5 * float(np.sum((xK - xt) ** 2))
 
-    for _ in range(8):
+    for _ in range(64):  # 64 seeded directions for each of y and x0
         for g, v, which in ((gy, y32, "y"), (gx0, x032, "x")):
             d = rng.standard_normal(v.size)
             h = 1e-5 * np.linalg.norm(v) / np.linalg.norm(d)
