@@ -609,7 +609,10 @@ __device__ __forceinline__ void load_dict_rounds(uint32_t rounds, const DictSlot
   }
 }
 
-template <int NB, int EPT, int VK, class Epi>
+// EXACT: lane t adds segment t's products (each one fp32-rounded) one by one
+// in element order — the reference's sequential sum — instead of the 16-lane
+// tree; a segment longer than a tile is walked by one lane.
+template <int NB, int EPT, int VK, bool EXACT, class Epi>
 __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
                                           const uint2* __restrict__ el,
                                           const uint4* __restrict__ tiles, uint32_t tile,
@@ -643,8 +646,12 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
   const uint32_t nseg = d1.x - s0, count = d1.y - e0, nwords = d1.w - g0;
   if (t < nseg) epi.prefetch(s0 + t);
   if (count > T) {
-    const float sum = long_segment_sum<NB, EPT>(el, v, e0, d1.y, red);
-    if (t == 0) epi.finish(s0, sum);
+    if constexpr (EXACT) {
+      if (t == 0) epi.finish(s0, seg_dot<1, true>(el, v, e0, d1.y, 0));
+    } else {
+      const float sum = long_segment_sum<NB, EPT>(el, v, e0, d1.y, red);
+      if (t == 0) epi.finish(s0, sum);
+    }
     return;
   }
   uint32_t send = 0;
@@ -695,6 +702,23 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
     }
   }
   __syncthreads();
+  if constexpr (EXACT) {
+    if (t < nseg) {
+      const uint32_t b = S[t], end = S[t + 1];
+      float acc = 0.f;
+      uint32_t k = b;
+      for (; k + 8 <= end; k += 8) {  // 8 reads issued, then 8 ordered adds
+        float p[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) p[j] = P[k + j];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc = acc + p[j];
+      }
+      for (; k < end; ++k) acc = acc + P[k];
+      epi.finish(s0 + t, acc);
+    }
+    return;
+  }
   const uint32_t lane = t % G, grp = t / G;
   for (uint32_t sg = grp; sg < nseg; sg += NB / G) {
     const uint32_t b = S[sg], end = S[sg + 1];
@@ -716,7 +740,7 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
   if (t < nseg) epi.finish(s0 + t, Sum[t]);
 }
 
-template <int NB, int EPT, int VK, bool RATIO, bool STATS>
+template <int NB, int EPT, int VK, bool EXACT, bool RATIO, bool STATS>
 __global__ __launch_bounds__(NB) void k_fwd_dict(
     const uint32_t* __restrict__ rowptr, const uint2* __restrict__ csr,
     const uint4* __restrict__ tiles, uint32_t ntiles, const void* __restrict__ dval,
@@ -728,11 +752,11 @@ __global__ __launch_bounds__(NB) void k_fwd_dict(
   const uint32_t tile = logical_block();
   FwdEpilogue<RATIO, STATS> epi{y, out, rscale};
   if (tile < ntiles)
-    dict_spmv<NB, EPT, VK>(rowptr, csr, tiles, tile, dval, dlix, dict, dmeta, x, cols, epi);
+    dict_spmv<NB, EPT, VK, EXACT>(rowptr, csr, tiles, tile, dval, dlix, dict, dmeta, x, cols, epi);
   if constexpr (STATS) stats_finish<NB>(epi.ll, epi.tot, partials, counter, results, ring);
 }
 
-template <int NB, int EPT, int VK, bool UPDATE>
+template <int NB, int EPT, int VK, bool EXACT, bool UPDATE>
 __global__ __launch_bounds__(NB) void k_bwd_dict(
     const uint32_t* __restrict__ colptr, const uint2* __restrict__ csc,
     const uint4* __restrict__ tiles, uint32_t ntiles, const void* __restrict__ dval,
@@ -743,7 +767,7 @@ __global__ __launch_bounds__(NB) void k_bwd_dict(
   const uint32_t tile = logical_block();
   if (tile >= ntiles) return;
   BwdEpilogue<UPDATE> epi{norm, out};
-  dict_spmv<NB, EPT, VK>(colptr, csc, tiles, tile, dval, dlix, dict, dmeta, r, rows, epi);
+  dict_spmv<NB, EPT, VK, EXACT>(colptr, csc, tiles, tile, dval, dlix, dict, dmeta, r, rows, epi);
 }
 
 // Packs every 64-entry dictionary slot in place, one wave per slot: when the
@@ -1148,65 +1172,86 @@ void check_launch(const char* what) {
   if (e != hipSuccess) raise(MLEM_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-template <int NB, int EPT, int VK>
-void fwd_dv(const DevMatrix& m, const float* x, const int32_t* y, float* out, bool ratio,
-            const StatsState* st, hipStream_t s) {
+template <int NB, int EPT, int VK, bool EXACT>
+void fwd_dvx(const DevMatrix& m, const float* x, const int32_t* y, float* out, bool ratio,
+             const StatsState* st, hipStream_t s) {
   const DictFormat& D = m.dfwd;
   const bool stats = ratio && st != nullptr;
   const dim3 grid(tile_grid(D.ntiles)), block(NB);
   if (stats && grid.x > st->max_blocks) raise(MLEM_ESTATE, "statistics buffer too small for launch");
   if (stats)
-    hipLaunchKernelGGL((k_fwd_dict<NB, EPT, VK, true, true>), grid, block, 0, s, m.rowptr, m.csr,
-                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, x, m.cols, y, out, m.rscale,
-                       st->partials, st->counter, st->results, st->ring);
+    hipLaunchKernelGGL((k_fwd_dict<NB, EPT, VK, EXACT, true, true>), grid, block, 0, s, m.rowptr,
+                       m.csr, D.tiles, D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, x, m.cols, y, out,
+                       m.rscale, st->partials, st->counter, st->results, st->ring);
   else if (ratio)
-    hipLaunchKernelGGL((k_fwd_dict<NB, EPT, VK, true, false>), grid, block, 0, s, m.rowptr, m.csr,
-                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, x, m.cols, y, out, m.rscale, nullptr,
-                       nullptr, nullptr, 0u);
+    hipLaunchKernelGGL((k_fwd_dict<NB, EPT, VK, EXACT, true, false>), grid, block, 0, s, m.rowptr,
+                       m.csr, D.tiles, D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, x, m.cols, y, out,
+                       m.rscale, nullptr, nullptr, nullptr, 0u);
   else
-    hipLaunchKernelGGL((k_fwd_dict<NB, EPT, VK, false, false>), grid, block, 0, s, m.rowptr, m.csr,
-                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, x, m.cols, y, out, m.rscale, nullptr,
-                       nullptr, nullptr, 0u);
+    hipLaunchKernelGGL((k_fwd_dict<NB, EPT, VK, EXACT, false, false>), grid, block, 0, s, m.rowptr,
+                       m.csr, D.tiles, D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, x, m.cols, y, out,
+                       m.rscale, nullptr, nullptr, nullptr, 0u);
+}
+
+// EXACT sums run on the 512-lane dictionary kernels only (kDictWide).
+template <int NB, int EPT, int VK>
+void fwd_dv(const DevMatrix& m, const float* x, const int32_t* y, float* out, bool ratio,
+            bool exact, const StatsState* st, hipStream_t s) {
+  if constexpr (NB == 512) {
+    if (exact) return fwd_dvx<NB, EPT, VK, true>(m, x, y, out, ratio, st, s);
+  }
+  if (exact) raise(MLEM_EINVAL, "EXACT dictionary kernels need the 512-lane family");
+  fwd_dvx<NB, EPT, VK, false>(m, x, y, out, ratio, st, s);
 }
 
 template <int NB, int EPT>
 void fwd_d(const DevMatrix& m, const float* x, const int32_t* y, float* out, bool ratio,
-           const StatsState* st, hipStream_t s) {
+           bool exact, const StatsState* st, hipStream_t s) {
   const DictFormat& D = m.dfwd;
   if (D.nb != NB || D.ept != EPT || !D.tiles)
     raise(MLEM_ESTATE, "forward dictionary not prepared");
   if (m.cols >= (1u << 30)) raise(MLEM_EINVAL, "dictionary kernels need cols < 2^30");
   switch (D.vk) {
-    case kF32: fwd_dv<NB, EPT, kF32>(m, x, y, out, ratio, st, s); break;
-    case kF16: fwd_dv<NB, EPT, kF16>(m, x, y, out, ratio, st, s); break;
-    default: fwd_dv<NB, EPT, kBF16>(m, x, y, out, ratio, st, s); break;
+    case kF32: fwd_dv<NB, EPT, kF32>(m, x, y, out, ratio, exact, st, s); break;
+    case kF16: fwd_dv<NB, EPT, kF16>(m, x, y, out, ratio, exact, st, s); break;
+    default: fwd_dv<NB, EPT, kBF16>(m, x, y, out, ratio, exact, st, s); break;
   }
+}
+
+template <int NB, int EPT, int VK, bool EXACT>
+void bwd_dvx(const DevMatrix& m, const float* r, const float* norm, float* out, bool update,
+             hipStream_t s) {
+  const DictFormat& D = m.dbwd;
+  const dim3 grid(tile_grid(D.ntiles)), block(NB);
+  if (update)
+    hipLaunchKernelGGL((k_bwd_dict<NB, EPT, VK, EXACT, true>), grid, block, 0, s, m.colptr, m.csc,
+                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, r, m.rows, norm, out);
+  else
+    hipLaunchKernelGGL((k_bwd_dict<NB, EPT, VK, EXACT, false>), grid, block, 0, s, m.colptr, m.csc,
+                       D.tiles, D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, r, m.rows, norm, out);
 }
 
 template <int NB, int EPT, int VK>
 void bwd_dv(const DevMatrix& m, const float* r, const float* norm, float* out, bool update,
-            hipStream_t s) {
-  const DictFormat& D = m.dbwd;
-  const dim3 grid(tile_grid(D.ntiles)), block(NB);
-  if (update)
-    hipLaunchKernelGGL((k_bwd_dict<NB, EPT, VK, true>), grid, block, 0, s, m.colptr, m.csc, D.tiles,
-                       D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, r, m.rows, norm, out);
-  else
-    hipLaunchKernelGGL((k_bwd_dict<NB, EPT, VK, false>), grid, block, 0, s, m.colptr, m.csc, D.tiles,
-                       D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, r, m.rows, norm, out);
+            bool exact, hipStream_t s) {
+  if constexpr (NB == 512) {
+    if (exact) return bwd_dvx<NB, EPT, VK, true>(m, r, norm, out, update, s);
+  }
+  if (exact) raise(MLEM_EINVAL, "EXACT dictionary kernels need the 512-lane family");
+  bwd_dvx<NB, EPT, VK, false>(m, r, norm, out, update, s);
 }
 
 template <int NB, int EPT>
 void bwd_d(const DevMatrix& m, const float* r, const float* norm, float* out, bool update,
-           hipStream_t s) {
+           bool exact, hipStream_t s) {
   const DictFormat& D = m.dbwd;
   if (D.nb != NB || D.ept != EPT || !D.tiles)
     raise(MLEM_ESTATE, "backward dictionary not prepared");
   if (m.rows >= (1u << 30)) raise(MLEM_EINVAL, "dictionary kernels need rows < 2^30");
   switch (D.vk) {
-    case kF32: bwd_dv<NB, EPT, kF32>(m, r, norm, out, update, s); break;
-    case kF16: bwd_dv<NB, EPT, kF16>(m, r, norm, out, update, s); break;
-    default: bwd_dv<NB, EPT, kBF16>(m, r, norm, out, update, s); break;
+    case kF32: bwd_dv<NB, EPT, kF32>(m, r, norm, out, update, exact, s); break;
+    case kF16: bwd_dv<NB, EPT, kF16>(m, r, norm, out, update, exact, s); break;
+    default: bwd_dv<NB, EPT, kBF16>(m, r, norm, out, update, exact, s); break;
   }
 }
 
@@ -1409,19 +1454,22 @@ void launch_fwd(const DevMatrix& m, int mode, const float* x, const int32_t* y, 
     stream_fwd(m, mode, x, y, out, ratio, stats, s);
     return;
   }
-  if (mode == MLEM_MODE_EXACT) {
+  // EXACT: the 512-lane dictionary kernels with per-lane ordered sums when
+  // they are configured, else one lane per row
+  const bool exact = mode == MLEM_MODE_EXACT;
+  if (exact && m.fwd.kind != kDictWide) {
     fwd_g<1, true>(m, x, y, out, ratio, stats, s);
   } else if (is_dict(m.fwd.kind)) {
     switch (dict_lanes(m.fwd.kind) * 1000 + m.fwd.ept) {
-      case 256008: fwd_d<256, 8>(m, x, y, out, ratio, stats, s); break;
-      case 256016: fwd_d<256, 16>(m, x, y, out, ratio, stats, s); break;
-      case 256024: fwd_d<256, 24>(m, x, y, out, ratio, stats, s); break;
-      case 256032: fwd_d<256, 32>(m, x, y, out, ratio, stats, s); break;
-      case 256048: fwd_d<256, 48>(m, x, y, out, ratio, stats, s); break;
-      case 256064: fwd_d<256, 64>(m, x, y, out, ratio, stats, s); break;
-      case 512016: fwd_d<512, 16>(m, x, y, out, ratio, stats, s); break;
-      case 512024: fwd_d<512, 24>(m, x, y, out, ratio, stats, s); break;
-      case 512032: fwd_d<512, 32>(m, x, y, out, ratio, stats, s); break;
+      case 256008: fwd_d<256, 8>(m, x, y, out, ratio, exact, stats, s); break;
+      case 256016: fwd_d<256, 16>(m, x, y, out, ratio, exact, stats, s); break;
+      case 256024: fwd_d<256, 24>(m, x, y, out, ratio, exact, stats, s); break;
+      case 256032: fwd_d<256, 32>(m, x, y, out, ratio, exact, stats, s); break;
+      case 256048: fwd_d<256, 48>(m, x, y, out, ratio, exact, stats, s); break;
+      case 256064: fwd_d<256, 64>(m, x, y, out, ratio, exact, stats, s); break;
+      case 512016: fwd_d<512, 16>(m, x, y, out, ratio, exact, stats, s); break;
+      case 512024: fwd_d<512, 24>(m, x, y, out, ratio, exact, stats, s); break;
+      case 512032: fwd_d<512, 32>(m, x, y, out, ratio, exact, stats, s); break;
       default: raise(MLEM_EINVAL, "unsupported forward dictionary tile size");
     }
   } else if (m.fwd.kind == kTiles) {
@@ -1451,19 +1499,20 @@ void launch_bwd(const DevMatrix& m, int mode, const float* r, const float* norm,
     return;
   }
   if (!m.csc) raise(MLEM_ESTATE, "matrix was opened without a CSC transpose");
-  if (mode == MLEM_MODE_EXACT) {
+  const bool exact = mode == MLEM_MODE_EXACT;
+  if (exact && m.bwd.kind != kDictWide) {
     bwd_g<1, true>(m, r, norm, out, update, s);
   } else if (is_dict(m.bwd.kind)) {
     switch (dict_lanes(m.bwd.kind) * 1000 + m.bwd.ept) {
-      case 256008: bwd_d<256, 8>(m, r, norm, out, update, s); break;
-      case 256016: bwd_d<256, 16>(m, r, norm, out, update, s); break;
-      case 256024: bwd_d<256, 24>(m, r, norm, out, update, s); break;
-      case 256032: bwd_d<256, 32>(m, r, norm, out, update, s); break;
-      case 256048: bwd_d<256, 48>(m, r, norm, out, update, s); break;
-      case 256064: bwd_d<256, 64>(m, r, norm, out, update, s); break;
-      case 512016: bwd_d<512, 16>(m, r, norm, out, update, s); break;
-      case 512024: bwd_d<512, 24>(m, r, norm, out, update, s); break;
-      case 512032: bwd_d<512, 32>(m, r, norm, out, update, s); break;
+      case 256008: bwd_d<256, 8>(m, r, norm, out, update, exact, s); break;
+      case 256016: bwd_d<256, 16>(m, r, norm, out, update, exact, s); break;
+      case 256024: bwd_d<256, 24>(m, r, norm, out, update, exact, s); break;
+      case 256032: bwd_d<256, 32>(m, r, norm, out, update, exact, s); break;
+      case 256048: bwd_d<256, 48>(m, r, norm, out, update, exact, s); break;
+      case 256064: bwd_d<256, 64>(m, r, norm, out, update, exact, s); break;
+      case 512016: bwd_d<512, 16>(m, r, norm, out, update, exact, s); break;
+      case 512024: bwd_d<512, 24>(m, r, norm, out, update, exact, s); break;
+      case 512032: bwd_d<512, 32>(m, r, norm, out, update, exact, s); break;
       default: raise(MLEM_EINVAL, "unsupported backward dictionary tile size");
     }
   } else if (m.bwd.kind == kTiles) {

@@ -272,6 +272,34 @@ def test_short_rows_fill_tiles_by_segment_count(family):
     A.close()
 
 
+@pytest.mark.parametrize("family", [("dict512", 16), ("dict512", 32), ("dict", 64)],
+                         ids=lambda f: f"{f[0]}{f[1]}")
+def test_exact_segments_longer_than_a_tile(family):
+    # a 20000-element row and a column shared by 9000 rows exceed every tile
+    # (8192 / 16384 elements): EXACT walks such a segment with one lane, the
+    # other segments with ordered per-lane sums over the tile's products
+    rng = np.random.default_rng(23)
+    cols = 30000
+    rows = [np.sort(rng.choice(np.arange(1, cols), 20000, replace=False))]
+    for _ in range(9000):
+        rows.append(np.concatenate([[0], np.sort(rng.choice(np.arange(1, cols),
+                                                             int(rng.integers(0, 12)),
+                                                             replace=False))]))
+    ends = np.cumsum([len(r) for r in rows]).astype(np.uint64)
+    el = np.zeros(int(ends[-1]), dtype=M.ELEMENT_DTYPE)
+    el["column"] = np.concatenate(rows).astype(np.uint32)
+    el["value"] = rng.uniform(0.1, 1.7, el.size).astype(np.float32)
+    A = M.Csr4Matrix.from_arrays(len(rows), cols, ends, el)
+    A.set_launch(family, family)
+    x = rng.uniform(0.1, 3, cols).astype(np.float32)
+    r = rng.uniform(0.1, 3, len(rows)).astype(np.float32)
+    assert np.array_equal(M.calcFwProj(A, x, "exact"), orc.fwproj(ends, el, x))
+    assert np.array_equal(M.calcBkProj(A, r, "exact"), orc.bkproj(ends, el, r, cols))
+    y = rng.integers(0, 30, len(rows)).astype(np.int32)
+    assert np.array_equal(M.mlem(A, y, None, 3, mode="exact"), orc.mlem(ends, el, cols, y, 3)[0])
+    A.close()
+
+
 def test_layout_with_partial_bricks():
     # 7x5x3 grid: bricks of 4x4x2 are cut at every face
     m = synth.make_matrix((7, 5, 3), 3000, 4242)
