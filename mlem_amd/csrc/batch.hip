@@ -367,37 +367,47 @@ void tune_widths(mlem_batch* B) {
   }
 }
 
-// Block width for A live frames: with measured costs, the width (<= wmax)
-// that minimises ceil(A/W) * cost(W); otherwise the narrowest power of two
-// that holds them (at most wmax).
-int choose_width(const mlem_batch* B, int A) {
+// Block widths for A live frames, widest first. With measured costs: the
+// blocks (each of width <= wmax) that hold A frames at the least summed
+// block-iteration cost, e.g. one 32-wide and one 8-wide block for 40 frames
+// rather than two 32-wide ones (dynamic programming over the frames
+// covered). Without: blocks of the narrowest power of two that holds them
+// (at most wmax).
+std::vector<int> choose_widths(const mlem_batch* B, int A) {
+  std::vector<int> ws;
+  if (A <= 0) return ws;
   if (B->wcost[0] > 0.0) {
-    int best = 4;
-    double bt = -1.0;
-    for (int i = 0, W = 4; i < 4 && W <= B->wmax; ++i, W *= 2) {
-      const double t = (double)((A + W - 1) / W) * B->wcost[i];
-      if (bt < 0.0 || t < bt) {
-        bt = t;
-        best = W;
+    std::vector<double> best((size_t)A + 1, -1.0);
+    std::vector<int> pick((size_t)A + 1, 0);
+    best[0] = 0.0;
+    for (int a = 1; a <= A; ++a)
+      for (int i = 0, W = 4; i < 4 && W <= B->wmax; ++i, W *= 2) {
+        const double c = best[(size_t)std::max(0, a - W)] + B->wcost[i];
+        if (best[(size_t)a] < 0.0 || c < best[(size_t)a]) {
+          best[(size_t)a] = c;
+          pick[(size_t)a] = W;
+        }
       }
-    }
-    return best;
+    for (int a = A; a > 0; a = std::max(0, a - pick[(size_t)a])) ws.push_back(pick[(size_t)a]);
+    std::sort(ws.rbegin(), ws.rend());
+    return ws;
   }
   int W = 4;
   while (W < B->wmax && W < A) W *= 2;
-  return W;
+  ws.assign((size_t)((A + W - 1) / W), W);
+  return ws;
 }
 
-// Place the still-active frames into the fewest blocks of the chosen width;
-// finished frames' images move to final_x.
+// Place the still-active frames into blocks of the chosen widths (frames in
+// index order, blocks widest first); finished frames' images move to final_x.
 void repack(mlem_batch* B, bool force) {
   DevMatrix& m = B->mat->m;
   int A = 0;
   for (uint32_t f = 0; f < B->F; ++f) A += B->active[f] ? 1 : 0;
-  const int W = choose_width(B, A);
-  const size_t nb = A == 0 ? 0 : (size_t)((A + W - 1) / W);
+  const std::vector<int> ws = choose_widths(B, A);
+  const size_t nb = ws.size();
   bool same = !force && nb == B->blocks.size();
-  for (auto& b : B->blocks) same = same && b->W == W;
+  for (size_t i = 0; same && i < nb; ++i) same = B->blocks[i]->W == ws[i];
   // save finished frames still living in a block
   for (auto& b : B->blocks)
     for (int l = 0; l < b->W; ++l) {
@@ -415,12 +425,16 @@ void repack(mlem_batch* B, bool force) {
   }
   drop_graph(B);
   std::vector<std::unique_ptr<RhsBlock>> nbk;
-  for (size_t i = 0; i < nb; ++i) nbk.push_back(new_block(B, W));
-  size_t slot = 0;
+  for (size_t i = 0; i < nb; ++i) nbk.push_back(new_block(B, ws[i]));
+  size_t bi = 0;
+  int l = 0;
   for (uint32_t f = 0; f < B->F; ++f) {
     if (!B->active[f]) continue;
-    RhsBlock& d = *nbk[slot / W];
-    const int l = (int)(slot % W);
+    if (l == nbk[bi]->W) {
+      ++bi;
+      l = 0;
+    }
+    RhsBlock& d = *nbk[bi];
     d.frames[l] = (int)f;
     // find the frame's current lane (if any)
     bool moved = false;
@@ -434,7 +448,7 @@ void repack(mlem_batch* B, bool force) {
           moved = true;
         }
     if (!moved) raise(MLEM_ESTATE, "batched frame lost during repacking");
-    ++slot;
+    ++l;
   }
   check_launch("repack");
   MLEM_HIP(hipStreamSynchronize(m.stream));
@@ -524,20 +538,23 @@ int mlem_batch_create(mlem_matrix* h, uint32_t frames, const int32_t* y, uint32_
     // initial blocks: every frame with a budget
     int A = 0;
     for (uint32_t f = 0; f < frames; ++f) A += B->active[f];
-    const int W = choose_width(B.get(), A);
-    const size_t nb = A == 0 ? 0 : (size_t)((A + W - 1) / W);
-    size_t slot = 0;
-    for (size_t i = 0; i < nb; ++i) B->blocks.push_back(new_block(B.get(), W));
+    const std::vector<int> ws = choose_widths(B.get(), A);
+    for (int W : ws) B->blocks.push_back(new_block(B.get(), W));
+    size_t bi = 0;
+    int l = 0;
     for (uint32_t f = 0; f < frames; ++f) {
       if (!B->active[f]) continue;
-      RhsBlock& b = *B->blocks[slot / W];
-      const int l = (int)(slot % W);
+      if (l == B->blocks[bi]->W) {
+        ++bi;
+        l = 0;
+      }
+      RhsBlock& b = *B->blocks[bi];
       b.frames[l] = (int)f;
       hipLaunchKernelGGL(k_lane_fill, dim3(blocks_for(m.cols)), dim3(kBlock), 0, m.stream, b.X,
                          b.W, l, x0[f], m.cols);
       hipLaunchKernelGGL(k_lane_y, dim3(blocks_for(m.rows)), dim3(kBlock), 0, m.stream, b.Y, b.W,
                          l, B->y_file + (size_t)f * m.rows, m.rowold, m.rows);
-      ++slot;
+      ++l;
     }
     check_launch("batch setup");
     for (auto& b : B->blocks) upload_mask(B.get(), *b);
