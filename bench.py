@@ -139,6 +139,14 @@ def cpu_baseline(problem, iters: int):
     out["openmp"] = {"value": 1.0 / omed, "unit": "iterations/s", "cores": threads,
                      "sample": f"{iters} iterations of the OpenMP restatement, median "
                                f"{omed:.3f} s per iteration"}
+    # thread scaling of the same restatement (2 iterations per thread count)
+    scaling = {}
+    t = 1
+    while t <= threads:
+        _, ts = orc.omp_mlem(m.rowidx, m.elements, m.cols, problem.y, 2, threads=t)
+        scaling[str(t)] = 1.0 / float(np.median(ts))
+        t *= 2
+    out["openmp"]["iterations_per_s_by_threads"] = scaling
     return out
 
 

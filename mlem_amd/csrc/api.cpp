@@ -171,6 +171,7 @@ struct mlem_recon {
   std::vector<double> loglik, fwdtot;  // per completed iteration
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  uint64_t graph_gen = 0;  // matrix launch generation the graph was captured at
   std::vector<hipEvent_t> events;
   ~mlem_recon() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -483,7 +484,14 @@ void run_iterations(mlem_recon* R, int iterations) {
     const int n =
         R->want_stats ? std::min<int>(left, (int)(kStatsRing / fwd_launches(m))) : left;
     if (R->use_graph) {
+      if (R->exec && R->graph_gen != m.launch_gen) {  // kernels or buffers changed
+        (void)hipGraphExecDestroy(R->exec);
+        (void)hipGraphDestroy(R->graph);
+        R->exec = nullptr;
+        R->graph = nullptr;
+      }
       if (!R->exec) {
+        R->graph_gen = m.launch_gen;
         MLEM_HIP(hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal));
         try {
           enqueue_iteration(R);
@@ -1160,6 +1168,7 @@ int mlem_set_launch(mlem_matrix* h, int fk, int fp, int bk, int bp) {
     if (!h) raise(MLEM_EINVAL, "null matrix handle");
     if (h->m.streamed) raise(MLEM_EINVAL, "streamed matrices have a fixed launch configuration");
     const GroupLaunch f = make_launch(fk, fp), b = make_launch(bk, bp);
+    ++h->m.launch_gen;
     h->m.fwd = f;
     h->m.bwd = b;
     prepare_tiles(h->m, h->m.stream);
@@ -1170,6 +1179,7 @@ int mlem_autotune(mlem_matrix* h) {
   return guarded([&] {
     if (!h) raise(MLEM_EINVAL, "null matrix handle");
     if (h->m.streamed) raise(MLEM_EINVAL, "streamed matrices have a fixed launch configuration");
+    ++h->m.launch_gen;
     autotune(h->m);
   });
 }

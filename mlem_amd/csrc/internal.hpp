@@ -145,6 +145,9 @@ struct DevMatrix {
   float* rscale = nullptr;     // [rows] device row order, nullptr for fp32
 
   GroupLaunch fwd, bwd, scatter;
+  // Bumped whenever the launch configuration (and with it the tile and
+  // dictionary buffers) may change; captured iterations are re-captured.
+  uint64_t launch_gen = 0;
 
   // Tiled kernels: device-order pointers kept on the host, and the tile
   // partition (first segment of every tile, plus the end) for the current

@@ -145,6 +145,23 @@ def test_recon_state_matches_one_shot(tiny):
     assert np.array_equal(R2.image(), full)
 
 
+def test_launch_change_between_runs_recaptures(tiny):
+    # a reconstruction whose iteration is captured in a graph keeps working
+    # after the matrix's kernels (and their tile buffers) are replaced
+    p, A = tiny
+    m = p.matrix
+    A.set_launch(("dict512", 32), ("dict512", 32))
+    R = M.Recon(A, p.y, mode="exact")
+    R.run(3)
+    A.set_launch(("dict", 16), ("tiles", 8))
+    R.run(2)
+    A.set_launch(("dict512", 16), ("dict512", 16))
+    R.run(2)
+    assert np.array_equal(R.image(), orc.mlem(m.rowidx, m.elements, m.cols, p.y, 7)[0])
+    R.close()
+    A.set_launch(("dict512", 32), ("dict512", 32))
+
+
 def test_stats_loglik_and_counts(tiny):
     p, A = tiny
     m = p.matrix
