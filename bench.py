@@ -150,7 +150,8 @@ def cpu_baseline(problem, iters: int):
     return out
 
 
-def streamed_run(A, prob, block_mib: int, iters: int, plain_value: float) -> dict:
+def streamed_run(A, prob, block_mib: int, iters: int, plain_value: float,
+                 resident_frac: float = 0.0) -> dict:
     """The same reconstruction with the matrix kept in pinned host memory and
     streamed to the device in row blocks of at most `block_mib` MiB per part
     (out-of-core path): iterations/s, host-to-device bytes per iteration
@@ -159,10 +160,12 @@ def streamed_run(A, prob, block_mib: int, iters: int, plain_value: float) -> dic
     import mlem_amd as M
     m = prob.matrix
     t0 = time.perf_counter()
+    resident = int(resident_frac * 16 * m.nnz)
     S = M.Csr4Matrix.from_arrays(m.rows, m.cols, m.rowidx, m.elements,
-                                 stream_block_bytes=block_mib << 20)
+                                 stream_block_bytes=block_mib << 20,
+                                 stream_resident_bytes=resident)
     open_s = time.perf_counter() - t0
-    nb = S.stream_blocks()
+    nb, nr = S.stream_blocks(), S.resident_blocks()
     R = M.Recon(S, prob.y, mode="fast")
     R.run(1)
     M.device_sync()
@@ -176,8 +179,8 @@ def streamed_run(A, prob, block_mib: int, iters: int, plain_value: float) -> dic
     xh = M.mlem(A, prob.y, None, iters + 1, mode="fast")
     d = np.abs(xs.astype(np.float64) - xh.astype(np.float64))
     floor = 1e-6 * float(np.max(np.abs(xh)))
-    moved = 16 * m.nnz + 4 * (m.rows + nb * (m.cols + 1))
-    return {"block_mib": block_mib, "blocks": nb, "iterations": iters,
+    moved = (16 * m.nnz + 4 * (m.rows + nb * (m.cols + 1))) * (nb - nr) / nb
+    return {"block_mib": block_mib, "blocks": nb, "resident_blocks": nr, "iterations": iters,
             "ms_per_iteration": 1e3 * wall / iters, "iterations_per_s": iters / wall,
             "h2d_bytes_per_iteration": moved, "h2d_gbs": moved * iters / wall / 1e9,
             "slowdown_vs_hbm": plain_value / (iters / wall), "open_s": open_s,
@@ -365,6 +368,8 @@ def main() -> int:
     if args.streamed_iters > 0 and args.precision == "fp32" and args.mode == "fast":
         try:
             streamed = streamed_run(A, prob, args.streamed_block_mib, args.streamed_iters, value)
+            streamed["half_resident"] = streamed_run(A, prob, args.streamed_block_mib,
+                                                     args.streamed_iters, value, 0.5)
         except Exception as e:  # reported in the line, never hides the plain result
             streamed = {"error": f"{type(e).__name__}: {e}"}
         print(f"[bench] streamed: {streamed}", file=sys.stderr, flush=True)

@@ -16,12 +16,13 @@
  *     not thread-safe; use one handle per host thread.
  *   - Matrix storage on the device: CSR {u32 column, f32 value} (the csr4
  *     element layout, 8 B) plus a CSC transpose {u32 row, f32 value} built on
- *     the device with rows kept in ascending order inside every column.
+ *     the device with rows kept in ascending order inside every column, and
+ *     per-tile dictionary formats of both built from them for the kernels.
  *
  * Compute modes (mlem_mode):
- *   MLEM_MODE_FAST   sub-wavefront row/column groups, fp32 fused multiply-add,
- *                    tree reductions; deterministic run to run.
- *   MLEM_MODE_EXACT  one lane per row (forward) and per column (backward),
+ *   MLEM_MODE_FAST   fp32 products summed in fixed trees per row (column);
+ *                    deterministic run to run.
+ *   MLEM_MODE_EXACT  each row's (column's) products summed one by one in
  *                    ascending element order, separate multiply and add,
  *                    flush-to-zero: reproduces the reference's fp32 rounding
  *                    sequence bit for bit.
@@ -129,9 +130,11 @@ MLEM_API int mlem_open_host(uint32_t rows, uint32_t cols, uint64_t nnz,
 /* Out-of-core matrices (larger than HBM). The elements stay in pinned host
  * memory in blocks of whole rows; a block holds its rows (CSR) and their
  * transpose (CSC), each part at most block_bytes bytes (0: a quarter of the
- * device memory free at open). Every forward projection streams the CSR parts
- * and every backprojection the CSC parts over PCIe through two device
- * buffers, the copy of one block overlapping the kernels of the block before.
+ * device memory free at open). The first blocks whose parts fit in
+ * resident_bytes are also kept in device memory (0: none). Every forward
+ * projection streams the CSR parts of the other blocks and every
+ * backprojection their CSC parts over PCIe through two device buffers, the
+ * copy of one block overlapping the kernels of the block before.
  * A column's backprojection continues its running sum from block to block in
  * ascending row order, so EXACT results equal the in-core ones bit for bit.
  * Every step function, mode and mlem_reconstruct / mlem_recon_* work on such a
@@ -144,14 +147,16 @@ MLEM_API int mlem_open_host(uint32_t rows, uint32_t cols, uint64_t nnz,
  * mlem_cuda/helper_files/helper_v_7_8.cu:4-72 (parts sized by free device
  * memory). */
 MLEM_API int mlem_open_streamed(const char* path, unsigned flags, uint64_t block_bytes,
-                                mlem_matrix** out);
+                                uint64_t resident_bytes, mlem_matrix** out);
 MLEM_API int mlem_open_host_streamed(uint32_t rows, uint32_t cols, uint64_t nnz,
                                      const uint64_t* rowidx, const void* elements,
                                      const uint32_t grid[3], unsigned flags,
-                                     uint64_t block_bytes, mlem_matrix** out);
-/* Row blocks of a streamed matrix; 0 for a matrix held in HBM. No reference
- * counterpart (version_7.cu prints its part count). */
-MLEM_API int mlem_stream_blocks(const mlem_matrix* m, uint32_t* blocks);
+                                     uint64_t block_bytes, uint64_t resident_bytes,
+                                     mlem_matrix** out);
+/* Row blocks of a streamed matrix and how many of them are resident in
+ * device memory (resident may be NULL); 0 for a matrix held in HBM. No
+ * reference counterpart (version_7.cu prints its part count). */
+MLEM_API int mlem_stream_blocks(const mlem_matrix* m, uint32_t* blocks, uint32_t* resident);
 
 /* Release device memory, stream and handle. Replaces Csr4Matrix::~Csr4Matrix
  * (csr4matrix.cpp:159-163). Accepts NULL. */

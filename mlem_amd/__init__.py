@@ -91,13 +91,14 @@ class Csr4Matrix:
     largest value) times 16-bit quotients; "fp32" keeps the file's values.
     stream_block_bytes (not None) keeps the matrix in pinned host memory and
     streams it to the device in row blocks of at most that many bytes per
-    part (0: sized from the free device memory), for matrices larger than HBM.
+    part (0: sized from the free device memory), for matrices larger than HBM;
+    the first blocks that fit stream_resident_bytes stay in device memory.
     """
 
     def __init__(self, path: Optional[str] = None, *, no_csc: bool = False,
                  keep_order: bool = False, precision: str = "fp32",
                  expand_symmetry: bool = False, stream_block_bytes: Optional[int] = None,
-                 _handle=None):
+                 stream_resident_bytes: int = 0, _handle=None):
         self._h = None
         self.precision = precision
         if _handle is not None:
@@ -113,7 +114,8 @@ class Csr4Matrix:
                 check(lib().mlem_open(os.fsencode(path), flags, C.byref(h)))
             else:
                 check(lib().mlem_open_streamed(os.fsencode(path), flags,
-                                               int(stream_block_bytes), C.byref(h)))
+                                               int(stream_block_bytes),
+                                               int(stream_resident_bytes), C.byref(h)))
             self._h = h
         r, c, n = C.c_uint32(), C.c_uint32(), C.c_uint64()
         check(lib().mlem_dims(self._h, C.byref(r), C.byref(c), C.byref(n)))
@@ -122,8 +124,8 @@ class Csr4Matrix:
     @classmethod
     def from_arrays(cls, rows: int, cols: int, rowidx: np.ndarray, elements: np.ndarray,
                     grid=None, no_csc: bool = False, keep_order: bool = False,
-                    precision: str = "fp32",
-                    stream_block_bytes: Optional[int] = None) -> "Csr4Matrix":
+                    precision: str = "fp32", stream_block_bytes: Optional[int] = None,
+                    stream_resident_bytes: int = 0) -> "Csr4Matrix":
         """grid = (nx, ny, nz) enables the device locality layout."""
         flags = _flags(no_csc, keep_order, precision)
         rowidx = np.ascontiguousarray(rowidx, dtype=np.uint64)
@@ -140,14 +142,21 @@ class Csr4Matrix:
                                        flags, C.byref(h)))
         else:
             check(lib().mlem_open_host_streamed(rows, cols, nnz, _ptr(rowidx), _ptr(elements),
-                                                g, flags, int(stream_block_bytes), C.byref(h)))
+                                                g, flags, int(stream_block_bytes),
+                                                int(stream_resident_bytes), C.byref(h)))
         return cls(_handle=h, precision=precision)
 
     def stream_blocks(self) -> int:
         """Row blocks of a streamed matrix (0 when the matrix is in HBM)."""
-        n = C.c_uint32()
-        check(lib().mlem_stream_blocks(self._h, C.byref(n)))
+        n, r = C.c_uint32(), C.c_uint32()
+        check(lib().mlem_stream_blocks(self._h, C.byref(n), C.byref(r)))
         return n.value
+
+    def resident_blocks(self) -> int:
+        """Blocks of a streamed matrix kept in device memory."""
+        n, r = C.c_uint32(), C.c_uint32()
+        check(lib().mlem_stream_blocks(self._h, C.byref(n), C.byref(r)))
+        return r.value
 
     def rows(self) -> int:
         return self._rows

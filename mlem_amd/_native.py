@@ -75,10 +75,10 @@ SIGNATURES = {
     "mlem_expand_symmetry_file": (C.c_int, [C.c_char_p, C.c_char_p]),
     "mlem_open": (C.c_int, [C.c_char_p, C.c_uint, C.POINTER(_P)]),
     "mlem_open_host": (C.c_int, [_u32, _u32, _u64, _P, _P, _P, C.c_uint, C.POINTER(_P)]),
-    "mlem_open_streamed": (C.c_int, [C.c_char_p, C.c_uint, _u64, C.POINTER(_P)]),
-    "mlem_open_host_streamed": (C.c_int, [_u32, _u32, _u64, _P, _P, _P, C.c_uint, _u64,
+    "mlem_open_streamed": (C.c_int, [C.c_char_p, C.c_uint, _u64, _u64, C.POINTER(_P)]),
+    "mlem_open_host_streamed": (C.c_int, [_u32, _u32, _u64, _P, _P, _P, C.c_uint, _u64, _u64,
                                           C.POINTER(_P)]),
-    "mlem_stream_blocks": (C.c_int, [_P, _PU32]),
+    "mlem_stream_blocks": (C.c_int, [_P, _PU32, _PU32]),
     "mlem_close": (None, [_P]),
     "mlem_dims": (C.c_int, [_P, _PU32, _PU32, C.POINTER(C.c_uint64)]),
     "mlem_geometry": (C.c_int, [_P, _PU32, _PU32]),

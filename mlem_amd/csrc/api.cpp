@@ -732,7 +732,7 @@ namespace {
 // seqcsr4mlem.cpp:37-49).
 mlem_matrix* open_streamed(uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t* rowidx,
                            const ElementFetch& fetch, const uint32_t grid[3], unsigned flags,
-                           uint64_t block_bytes) {
+                           uint64_t block_bytes, uint64_t resident_bytes) {
   if (flags & (MLEM_OPEN_FP16 | MLEM_OPEN_BF16))
     raise(MLEM_EINVAL, "streamed matrices hold fp32 values");
   if (rows == 0xFFFFFFFFu || cols == 0xFFFFFFFFu) raise(MLEM_EINVAL, "dimension too large");
@@ -746,7 +746,8 @@ mlem_matrix* open_streamed(uint32_t rows, uint32_t cols, uint64_t nnz, const uin
   select_device(&m.device, &m.cus);
   MLEM_HIP(hipStreamCreateWithFlags(&m.stream, hipStreamNonBlocking));
   auto t0 = std::chrono::steady_clock::now();
-  m.streamed = build_streamed(rows, cols, nnz, rowidx, fetch, block_bytes, m.stream);
+  m.streamed =
+      build_streamed(rows, cols, nnz, rowidx, fetch, block_bytes, resident_bytes, m.stream);
   h->load_seconds = seconds_since(t0);
   t0 = std::chrono::steady_clock::now();
   MLEM_HIP(hipMalloc(&m.norm, 4ull * std::max<uint32_t>(cols, 1)));
@@ -770,7 +771,7 @@ mlem_matrix* open_streamed(uint32_t rows, uint32_t cols, uint64_t nnz, const uin
 }  // namespace
 
 int mlem_open_streamed(const char* path, unsigned flags, uint64_t block_bytes,
-                       mlem_matrix** out) {
+                       uint64_t resident_bytes, mlem_matrix** out) {
   return guarded([&] {
     if (!path || !out) raise(MLEM_EINVAL, "null argument");
     *out = nullptr;
@@ -806,7 +807,7 @@ int mlem_open_streamed(const char* path, unsigned flags, uint64_t block_bytes,
           [&](void* dst, uint64_t first, uint64_t count) {
             std::memcpy(dst, src + 8 * first, 8 * count);
           },
-          hd.scanner.grid, flags, block_bytes));
+          hd.scanner.grid, flags, block_bytes, resident_bytes));
     } else {
       h.reset(open_streamed(
           hd.nRows, hd.nColumns, hd.nnz, rowidx.data(),
@@ -820,7 +821,7 @@ int mlem_open_streamed(const char* path, unsigned flags, uint64_t block_bytes,
               done += (uint64_t)got;
             }
           },
-          hd.scanner.grid, flags, block_bytes));
+          hd.scanner.grid, flags, block_bytes, resident_bytes));
       h->m.symflags = hd.flags & 7u;
     }
     *out = h.release();
@@ -829,7 +830,7 @@ int mlem_open_streamed(const char* path, unsigned flags, uint64_t block_bytes,
 
 int mlem_open_host_streamed(uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t* rowidx,
                             const void* elements, const uint32_t grid[3], unsigned flags,
-                            uint64_t block_bytes, mlem_matrix** out) {
+                            uint64_t block_bytes, uint64_t resident_bytes, mlem_matrix** out) {
   return guarded([&] {
     if (!out || (rows && !rowidx) || (nnz && !elements)) raise(MLEM_EINVAL, "null argument");
     *out = nullptr;
@@ -840,14 +841,20 @@ int mlem_open_host_streamed(uint32_t rows, uint32_t cols, uint64_t nnz, const ui
         [&](void* dst, uint64_t first, uint64_t count) {
           std::memcpy(dst, src + 8 * first, 8 * count);
         },
-        grid, flags, block_bytes);
+        grid, flags, block_bytes, resident_bytes);
   });
 }
 
-int mlem_stream_blocks(const mlem_matrix* h, uint32_t* blocks) {
+int mlem_stream_blocks(const mlem_matrix* h, uint32_t* blocks, uint32_t* resident) {
   return guarded([&] {
     if (!h || !blocks) raise(MLEM_EINVAL, "null argument");
-    *blocks = h->m.streamed ? (uint32_t)h->m.streamed->blocks.size() : 0u;
+    const StreamedMatrix* S = h->m.streamed;
+    *blocks = S ? (uint32_t)S->blocks.size() : 0u;
+    if (resident) {
+      *resident = 0;
+      if (S)
+        for (const StreamBlock& B : S->blocks) *resident += B.resident ? 1u : 0u;
+    }
   });
 }
 

@@ -84,6 +84,17 @@ int main(int argc, char* argv[]) {
       block_bytes = (uint64_t)mib << 20;
     }
   }
+  // MLEM_STREAM_RESIDENT=<MiB>: of a streamed matrix, keep that many MiB of
+  // blocks in device memory
+  uint64_t resident_bytes = 0;
+  if (const char* s = std::getenv("MLEM_STREAM_RESIDENT")) {
+    if (*s) {
+      int mib = 0;
+      if (parse_int(s, "MLEM_STREAM_RESIDENT (MiB)", &mib)) return 1;
+      if (mib < 0) return fail("MLEM_STREAM_RESIDENT must be >= 0");
+      resident_bytes = (uint64_t)mib << 20;
+    }
+  }
 
   std::cout << "Matrix file: " << mtx << std::endl;
   std::cout << "Input file: " << in << std::endl;
@@ -100,7 +111,7 @@ int main(int argc, char* argv[]) {
 
   mlem_matrix* m = nullptr;
   const int orc = streamed ? mlem_open_streamed(mtx.c_str(), open_flags & MLEM_OPEN_EXPAND_SYMMETRY,
-                                                 block_bytes, &m)
+                                                 block_bytes, resident_bytes, &m)
                            : mlem_open(mtx.c_str(), open_flags, &m);
   if (orc != MLEM_OK) return fail(mlem_last_error());
   uint32_t rows = 0, cols = 0;

@@ -98,6 +98,7 @@ struct StreamBlock {
   uint64_t nnz = 0;
   uint64_t csr_off = 0, csr_bytes = 0;  // parts within the host buffer
   uint64_t csc_off = 0, csc_bytes = 0;
+  char* resident = nullptr;  // device copy of both parts (CSR, then CSC), or none
 };
 struct StreamedMatrix {
   char* host = nullptr;  // pinned: every block's CSR part, then its CSC part
@@ -213,7 +214,7 @@ void launch_bwd_block_add(const uint32_t* colptr, const uint2* csc, uint32_t col
 using ElementFetch = std::function<void(void* dst, uint64_t first, uint64_t count)>;
 StreamedMatrix* build_streamed(uint32_t rows, uint32_t cols, uint64_t nnz,
                                const uint64_t* rowidx, const ElementFetch& fetch,
-                               uint64_t block_bytes, hipStream_t s);
+                               uint64_t block_bytes, uint64_t resident_bytes, hipStream_t s);
 void stream_fwd(const DevMatrix& m, int mode, const float* x, const int32_t* y, float* out,
                 bool ratio, const StatsState* st, hipStream_t s);
 void stream_bwd(const DevMatrix& m, int mode, const float* r, const float* norm, float* out,

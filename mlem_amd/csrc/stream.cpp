@@ -32,6 +32,8 @@ constexpr uint64_t kMaxBlockNnz = (1ull << 32) - 1024;
 }  // namespace
 
 StreamedMatrix::~StreamedMatrix() {
+  for (StreamBlock& b : blocks)
+    if (b.resident) (void)hipFree(b.resident);
   for (int i = 0; i < 2; ++i) {
     if (copied[i]) (void)hipEventDestroy(copied[i]);
     if (freed[i]) (void)hipEventDestroy(freed[i]);
@@ -44,7 +46,7 @@ StreamedMatrix::~StreamedMatrix() {
 
 StreamedMatrix* build_streamed(uint32_t rows, uint32_t cols, uint64_t nnz,
                                const uint64_t* rowidx, const ElementFetch& fetch,
-                               uint64_t block_bytes, hipStream_t s) {
+                               uint64_t block_bytes, uint64_t resident_bytes, hipStream_t s) {
   const uint64_t colpart = align16(4ull * ((uint64_t)cols + 1));
   if (block_bytes == 0) {
     // a quarter of the free device memory per buffer, as the reference sizes
@@ -145,6 +147,23 @@ StreamedMatrix* build_streamed(uint32_t rows, uint32_t cols, uint64_t nnz,
     raise(MLEM_EFORMAT, std::to_string(bad.load()) + " matrix elements have a column index >= " +
                             std::to_string(cols));
 
+  // Blocks kept in device memory: the first ones whose two parts fit the
+  // resident budget; only the others are streamed.
+  uint64_t kept = 0, streamed_part = 0;
+  for (StreamBlock& B : S->blocks) {
+    const uint64_t both = align16(B.csr_bytes) + B.csc_bytes;
+    if (kept + both <= resident_bytes && streamed_part == 0) {
+      MLEM_HIP(hipMalloc((void**)&B.resident, both));
+      MLEM_HIP(hipMemcpyAsync(B.resident, S->host + B.csr_off, B.csr_bytes,
+                              hipMemcpyHostToDevice, s));
+      MLEM_HIP(hipMemcpyAsync(B.resident + align16(B.csr_bytes), S->host + B.csc_off,
+                              B.csc_bytes, hipMemcpyHostToDevice, s));
+      kept += both;
+    } else {
+      streamed_part = std::max(streamed_part, std::max(B.csr_bytes, B.csc_bytes));
+    }
+  }
+  S->part_bytes = streamed_part;
   for (int i = 0; i < 2; ++i) {
     MLEM_HIP(hipMalloc((void**)&S->dev[i], std::max<uint64_t>(S->part_bytes, 16)));
     MLEM_HIP(hipEventCreateWithFlags(&S->copied[i], hipEventDisableTiming));
@@ -176,9 +195,14 @@ namespace {
 template <class K>
 void stream_pass(const DevMatrix& m, bool csc, K&& kernel, hipStream_t s) {
   StreamedMatrix& S = *m.streamed;
+  size_t n_streamed = 0;
   for (size_t b = 0; b < S.blocks.size(); ++b) {
-    const int i = (int)(b & 1);
     const StreamBlock& B = S.blocks[b];
+    if (B.resident) {  // in device memory: no copy
+      kernel(B, csc ? B.resident + align16(B.csr_bytes) : B.resident);
+      continue;
+    }
+    const int i = (int)(n_streamed++ & 1);
     const uint64_t off = csc ? B.csc_off : B.csr_off;
     const uint64_t n = csc ? B.csc_bytes : B.csr_bytes;
     MLEM_HIP(hipStreamWaitEvent(S.copy, S.freed[i], 0));

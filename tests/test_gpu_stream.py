@@ -77,6 +77,25 @@ def test_fast_and_atomic_vs_fp64(small_streamed, mode):
     assert np.all(np.abs(tot[1:] - ysum) <= 1e-5 * ysum)
 
 
+@pytest.mark.parametrize("resident_mib", [24, 1 << 12])
+def test_resident_blocks_exact(resident_mib):
+    # some blocks (or all) kept in device memory, the rest streamed: the same
+    # bit-exact results
+    p = synth.make_problem("small")
+    m = p.matrix
+    A = M.Csr4Matrix.from_arrays(m.rows, m.cols, m.rowidx, m.elements,
+                                 stream_block_bytes=8 << 20,
+                                 stream_resident_bytes=resident_mib << 20)
+    nb, nr = A.stream_blocks(), A.resident_blocks()
+    assert 0 < nr <= nb and (nr < nb) == (resident_mib == 24)
+    assert np.array_equal(M.mlem(A, p.y, None, 4, mode="exact"),
+                          orc.mlem(m.rowidx, m.elements, m.cols, p.y, 4)[0])
+    xf, ll, _ = M.mlem(A, p.y, None, 4, mode="fast", stats=True)
+    x64, ll64, _ = orc.f64_mlem(m.rowidx, m.elements, m.cols, p.y.astype(np.float64), 4)
+    assert rel_linf(xf, x64) < TOL_RECON and rel_linf(ll, ll64) < 1e-6
+    A.close()
+
+
 def test_recon_incremental_matches_one_shot(small_streamed):
     p, A = small_streamed
     R = M.Recon(A, p.y, mode="exact")
