@@ -105,6 +105,38 @@ def test_recon_incremental_matches_one_shot(small_streamed):
     R.close()
 
 
+def test_timed_run(small_streamed):
+    # per-kernel HIP-event timing covers every block's copy and kernels
+    p, A = small_streamed
+    R = M.Recon(A, p.y, mode="exact")
+    kf, kb, span = R.run_timed(3)
+    assert np.all(kf > 0) and np.all(kb > 0) and span >= kf.sum()
+    assert np.array_equal(R.image(), M.mlem(A, p.y, None, 3, mode="exact"))
+    R.close()
+
+
+def test_cli_streamed_checkpoint_restart(tmp_path, tiny_problem):
+    # the 5th argument (checkpoint interval) with a streamed matrix: restart
+    # from img.chkpt / iter.chkpt gives the uninterrupted result
+    p = tiny_problem
+    m = p.matrix
+    mtx, sino = tmp_path / "m.csr4", tmp_path / "s.sino"
+    m.write_csr4(str(mtx))
+    p.y.astype(np.int32).tofile(sino)
+    env = dict(os.environ, MLEM_MODE="exact", MLEM_STREAM="0")
+    cli = str(ROOT / "mlem_amd" / "bin" / "mlem_cli")
+    xo3 = orc.mlem(m.rowidx, m.elements, m.cols, p.y, 3)[0]
+    xo3.tofile(tmp_path / "img.chkpt")
+    np.int32([3]).tofile(tmp_path / "iter.chkpt")
+    out = tmp_path / "restart.img"
+    r = subprocess.run([cli, str(mtx), str(sino), str(out), "7", "2"], cwd=tmp_path, env=env,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "I got Checkpoint at iteration 3" in r.stdout
+    assert np.array_equal(np.fromfile(out, np.float32),
+                          orc.mlem(m.rowidx, m.elements, m.cols, p.y, 7)[0])
+
+
 def test_batched_and_unrolled_refuse_streamed(small_streamed):
     p, A = small_streamed
     with pytest.raises(MlemError):
