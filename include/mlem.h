@@ -275,10 +275,13 @@ MLEM_API int mlem_recon_iterations(const mlem_recon* r);
 typedef struct mlem_batch mlem_batch;
 typedef struct mlem_batch_options {
   double rel_tol;   /* early-stop tolerance on the log-likelihood (0 = off) */
-  int block_width;  /* widest RHS block: 4, 8, 16 or 32 frames; 0 =
-                       autotune: one block-iteration is timed at every width
-                       at creation and live frames are packed into the
-                       blocks that minimise blocks x measured time        */
+  int block_width;  /* widest RHS block: 4, 8, 16 or 32 frames;
+                       1 = every frame alone on the single-image
+                       MLEM_MODE_EXACT kernels;
+                       0 = autotune: one iteration is timed at every width
+                       (and alone) at creation and live frames are packed
+                       into the blocks that minimise the summed measured
+                       time, single frames included                       */
   int use_graph;    /* 1: replay each iteration from a captured hipGraph  */
 } mlem_batch_options;
 MLEM_API void mlem_batch_options_default(mlem_batch_options* o);
@@ -299,6 +302,10 @@ MLEM_API int mlem_batch_progress(const mlem_batch* b, uint32_t* active_frames,
 /* Measured milliseconds of one block-iteration at widths 4, 8, 16, 32
  * (zeros unless the batch was created with block_width 0). */
 MLEM_API int mlem_batch_tuning(const mlem_batch* b, double block_ms[4]);
+/* Measured milliseconds of one iteration of a lone frame on the single-image
+ * EXACT kernels, the cost of a width-1 block (0 unless block_width was 0).
+ * No reference counterpart. */
+MLEM_API int mlem_batch_single_ms(const mlem_batch* b, double* ms);
 MLEM_API void mlem_batch_destroy(mlem_batch* b);
 
 /* ---- differentiable unrolled MLEM ----------------------------------------

@@ -397,10 +397,13 @@ class Batch:
                 "blocks": nb.value}
 
     def tuning(self) -> dict:
-        """Measured ms per block-iteration by width (block_width=0 batches)."""
+        """Measured ms per block-iteration by width (block_width=0 batches);
+        width 1 is a lone frame on the single-image EXACT kernels."""
         ms = (C.c_double * 4)()
         check(lib().mlem_batch_tuning(self._h, ms))
-        return {w: ms[i] for i, w in enumerate((4, 8, 16, 32))}
+        one = C.c_double()
+        check(lib().mlem_batch_single_ms(self._h, C.byref(one)))
+        return {1: one.value, **{w: ms[i] for i, w in enumerate((4, 8, 16, 32))}}
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None:

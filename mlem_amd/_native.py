@@ -109,6 +109,7 @@ SIGNATURES = {
     "mlem_batch_progress": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
                                       C.POINTER(C.c_int), C.POINTER(C.c_uint32)]),
     "mlem_batch_tuning": (C.c_int, [_P, _P]),
+    "mlem_batch_single_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "mlem_batch_destroy": (None, [_P]),
     "mlem_forward_k": (C.c_int, [_P, _u32, _P, _u32, _P, _u32, C.c_int, C.c_int, _P,
                                  C.POINTER(_P)]),

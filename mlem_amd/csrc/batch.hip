@@ -17,6 +17,10 @@
 // deterministic two-level reduction; the host stops a frame when its budget
 // is spent or the relative log-likelihood change drops below the tolerance,
 // freezes it, and repacks the remaining frames into fewer / narrower blocks.
+// An autotuned batch may also give a frame a block of its own (width 1): it
+// then runs on the single-image kernels in EXACT mode, whose arithmetic per
+// frame is the same sequential fp32 one, so the few frames left at the end
+// of a dynamic series do not pay for a whole block each.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -42,16 +46,15 @@ void check_launch(const char* what) {
   if (e != hipSuccess) raise(MLEM_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// Deterministic per-frame FP64 reduction of (ll, tot) over the grid: block
-// partials, then the last block to finish sums them in a fixed order.
+// Deterministic per-frame FP64 reduction of (ll, tot), first stage: every
+// workgroup sums its lane groups in a fixed order and stores one partial per
+// frame. The second stage (k_batch_stats_reduce) is the next kernel in the
+// stream, so no device-scope fence or atomic runs in the projection kernel.
 template <int W>
-__device__ void batch_stats_finish(double ll, double tot, double2* __restrict__ partials,
-                                   unsigned* __restrict__ counter, double2* __restrict__ results,
-                                   unsigned ring) {
+__device__ void batch_stats_partial(double ll, double tot, double2* __restrict__ partials) {
   constexpr int G = kBlock / W;
   __shared__ double s_ll[kBlock], s_tot[kBlock];
-  __shared__ int s_last;
-  const uint32_t t = threadIdx.x, f = t % W, g = t / W;
+  const uint32_t t = threadIdx.x, f = t % W;
   s_ll[t] = ll;
   s_tot[t] = tot;
   __syncthreads();
@@ -61,42 +64,39 @@ __device__ void batch_stats_finish(double ll, double tot, double2* __restrict__ 
       a += s_ll[k * W + f];
       b += s_tot[k * W + f];
     }
-    __hip_atomic_store(&partials[(size_t)blockIdx.x * W + f].x, a, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&partials[(size_t)blockIdx.x * W + f].y, b, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    partials[(size_t)blockIdx.x * W + f] = make_double2(a, b);
   }
-  __threadfence();
-  __syncthreads();
-  if (t == 0) s_last = (atomicAdd(&counter[0], 1u) == gridDim.x - 1);
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
+}
+
+// Second stage, one workgroup: per frame, the sum of the n workgroup
+// partials in a fixed order, appended to the ring at slot counter[1].
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_batch_stats_reduce(
+    const double2* __restrict__ partials, unsigned n, unsigned* __restrict__ counter,
+    double2* __restrict__ results, unsigned ring) {
+  constexpr int G = kBlock / W;
+  __shared__ double s_ll[kBlock], s_tot[kBlock];
+  const uint32_t t = threadIdx.x, f = t % W, g = t / W;
   double a = 0.0, b = 0.0;
-  for (uint32_t blk = g; blk < gridDim.x; blk += G) {
-    a += __hip_atomic_load(&partials[(size_t)blk * W + f].x, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    b += __hip_atomic_load(&partials[(size_t)blk * W + f].y, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+  for (uint32_t blk = g; blk < n; blk += G) {
+    const double2 p = partials[(size_t)blk * W + f];
+    a += p.x;
+    b += p.y;
   }
-  __syncthreads();
   s_ll[t] = a;
   s_tot[t] = b;
   __syncthreads();
+  const unsigned slot = counter[1];
   if (t < (uint32_t)W) {
     double sa = 0.0, sb = 0.0;
     for (int k = 0; k < G; ++k) {
       sa += s_ll[k * W + f];
       sb += s_tot[k * W + f];
     }
-    const unsigned slot = counter[1] % ring;
-    results[(size_t)slot * W + f] = make_double2(sa, sb);
+    results[(size_t)(slot % ring) * W + f] = make_double2(sa, sb);
   }
-  __syncthreads();
-  if (t == 0) {
-    atomicAdd(&counter[1], 1u);
-    atomicExch(&counter[0], 0u);
-  }
+  __syncthreads();  // every lane has read the slot before it advances
+  if (t == 0) counter[1] = slot + 1;
 }
 
 // Forward epilogue: ratio r = (f != 0) ? y/f : 0 and the frame's FP64
@@ -136,11 +136,10 @@ __global__ __launch_bounds__(kBlock) void k_fwd_batch(
     const uint32_t* __restrict__ rowptr, const uint2* __restrict__ csr,
     const uint2* __restrict__ tiles, uint32_t ntiles,
     const float* __restrict__ X, const int32_t* __restrict__ Y, float* __restrict__ R,
-    double2* __restrict__ partials, unsigned* __restrict__ counter,
-    double2* __restrict__ results, unsigned ring) {
+    double2* __restrict__ partials) {
   FwdRatio<W> epi{Y, R};
   spmm::walk<W>(rowptr, csr, tiles, ntiles, X, epi);
-  batch_stats_finish<W>(epi.ll, epi.tot, partials, counter, results, ring);
+  batch_stats_partial<W>(epi.ll, epi.tot, partials);
 }
 
 // Backprojection + update for one RHS block over the CSC column tiles.
@@ -218,6 +217,7 @@ struct RhsBlock {
   float* R = nullptr;       // [rows][W]
   uint32_t* mask = nullptr; // bit f set: lane f still iterating
   double2* partials = nullptr;
+  unsigned pmax = 0;           // partials held, in workgroups (width 1)
   unsigned* counter = nullptr;
   double2* results = nullptr;  // [kRing][W]
   unsigned iters = 0;          // iterations run by this block (ring slots used)
@@ -241,12 +241,17 @@ struct mlem_batch {
   // measured cost (ms) of one block-iteration at widths 4, 8, 16, 32
   // (autotuned batches; 0 = not measured)
   double wcost[4] = {0, 0, 0, 0};
+  // measured cost (ms) of one iteration of a lone frame on the single-image
+  // EXACT kernels (autotuned batches; 0 = never used)
+  double wcost1 = 0.0;
+  uint64_t graph_gen = 0;  // matrix launch generation the graph was captured at
   double tol = 0.0;
   bool use_graph = false;
   std::vector<int> budget, done;
   std::vector<std::vector<double>> ll;   // per frame: log-likelihood of x_{k-1}, k = 1..done
   std::vector<char> active;
   std::vector<std::unique_ptr<RhsBlock>> blocks;
+  std::vector<std::unique_ptr<RhsBlock>> spare;  // blocks kept for reuse by repacking
   float* final_x = nullptr;    // [F][cols] file order: images of finished frames
   int32_t* y_file = nullptr;   // [F][rows] file order (kept for repacking)
   uint2* tiles_f = nullptr;
@@ -285,13 +290,41 @@ std::unique_ptr<RhsBlock> new_block(mlem_batch* B, int W) {
   MLEM_HIP(hipMalloc(&b->Y, 4ull * m.rows * W));
   MLEM_HIP(hipMalloc(&b->R, 4ull * m.rows * W));
   MLEM_HIP(hipMalloc(&b->mask, 4));
-  MLEM_HIP(hipMalloc(&b->partials, sizeof(double2) * (size_t)B->grid * W));
+  // width 1 runs the single-image forward kernel, whose grid follows the
+  // matrix's launch configuration
+  b->pmax = W == 1 ? std::max(B->grid, stats_max_blocks(m)) : B->grid * W;
+  MLEM_HIP(hipMalloc(&b->partials, sizeof(double2) * (size_t)b->pmax));
   MLEM_HIP(hipMalloc(&b->counter, 2 * sizeof(unsigned)));
   MLEM_HIP(hipMalloc(&b->results, sizeof(double2) * (size_t)kRing * W));
   MLEM_HIP(hipMemsetAsync(b->counter, 0, 2 * sizeof(unsigned), m.stream));
   MLEM_HIP(hipMemsetAsync(b->X, 0, 4ull * m.cols * W, m.stream));
   MLEM_HIP(hipMemsetAsync(b->Y, 0, 4ull * m.rows * W, m.stream));
   return b;
+}
+
+// A block of width W for repacking: a spare one when the batch holds one
+// (idle lanes may keep an earlier frame's values; the lane mask keeps the
+// update off them and their statistics are never read), else a new one.
+// Repacking then costs no device allocation or free per finished frame.
+std::unique_ptr<RhsBlock> take_block(mlem_batch* B, int W) {
+  const DevMatrix& m = B->mat->m;
+  for (size_t i = 0; i < B->spare.size(); ++i) {
+    if (B->spare[i]->W != W) continue;
+    std::unique_ptr<RhsBlock> b = std::move(B->spare[i]);
+    B->spare.erase(B->spare.begin() + (std::ptrdiff_t)i);
+    b->frames.assign(W, -1);
+    b->iters = 0;
+    MLEM_HIP(hipMemsetAsync(b->counter, 0, 2 * sizeof(unsigned), m.stream));
+    const unsigned need = stats_max_blocks(m);
+    if (W == 1 && need > b->pmax) {
+      MLEM_HIP(hipStreamSynchronize(m.stream));
+      dfree(b->partials);
+      MLEM_HIP(hipMalloc(&b->partials, sizeof(double2) * (size_t)need));
+      b->pmax = need;
+    }
+    return b;
+  }
+  return new_block(B, W);
 }
 
 void upload_mask(mlem_batch* B, RhsBlock& b) {
@@ -306,15 +339,33 @@ template <int W>
 void launch_block(mlem_batch* B, RhsBlock& b, hipStream_t s) {
   const DevMatrix& m = B->mat->m;
   hipLaunchKernelGGL((k_fwd_batch<W>), dim3(B->grid), dim3(kBlock), 0, s, m.rowptr, m.csr,
-                     B->tiles_f, B->nt_f, b.X, b.Y, b.R, b.partials, b.counter, b.results,
-                     kRing);
+                     B->tiles_f, B->nt_f, b.X, b.Y, b.R, b.partials);
+  hipLaunchKernelGGL((k_batch_stats_reduce<W>), dim3(1), dim3(kBlock), 0, s, b.partials, B->grid,
+                     b.counter, b.results, kRing);
   hipLaunchKernelGGL((k_bwd_batch<W>), dim3(B->grid), dim3(kBlock), 0, s, m.colptr, m.csc,
                      B->tiles_b, B->nt_b, b.R, m.norm, b.X, b.mask);
+}
+
+// A block of one frame: the single-image EXACT projections (forward with the
+// fused ratio and the frame's FP64 statistics into the block's ring,
+// backprojection with the fused update). X, Y and R of a width-1 block are
+// the plain device-order vectors these kernels take.
+void launch_single(mlem_batch* B, RhsBlock& b, hipStream_t s) {
+  const DevMatrix& m = B->mat->m;
+  StatsState st;
+  st.partials = b.partials;
+  st.counter = b.counter;
+  st.results = b.results;
+  st.ring = kRing;
+  st.max_blocks = b.pmax;
+  launch_fwd(m, MLEM_MODE_EXACT, b.X, b.Y, b.R, true, &st, s);
+  launch_bwd(m, MLEM_MODE_EXACT, b.R, m.norm, b.X, true, s);
 }
 
 void enqueue_iteration(mlem_batch* B, hipStream_t s) {
   for (auto& bp : B->blocks) {
     switch (bp->W) {
+      case 1: launch_single(B, *bp, s); break;
       case 4: launch_block<4>(B, *bp, s); break;
       case 8: launch_block<8>(B, *bp, s); break;
       case 16: launch_block<16>(B, *bp, s); break;
@@ -339,7 +390,9 @@ void tune_widths(mlem_batch* B) {
       (void)hipEventDestroy(b);
     }
   } ev{a, b};
-  for (int i = 0, W = 4; i < 4; ++i, W *= 2) {
+  static const int widths[5] = {1, 4, 8, 16, 32};
+  for (int k = 0; k < 5; ++k) {
+    const int W = widths[k];
     std::unique_ptr<RhsBlock> blk = new_block(B, W);
     for (int l = 0; l < W; ++l)
       hipLaunchKernelGGL(k_lane_fill, dim3(blocks_for(m.cols)), dim3(kBlock), 0, m.stream, blk->X,
@@ -350,6 +403,7 @@ void tune_widths(mlem_batch* B) {
     for (int r = 0; r < 4; ++r) {
       MLEM_HIP(hipEventRecord(a, m.stream));
       switch (W) {
+        case 1: launch_single(B, *blk, m.stream); break;
         case 4: launch_block<4>(B, *blk, m.stream); break;
         case 8: launch_block<8>(B, *blk, m.stream); break;
         case 16: launch_block<16>(B, *blk, m.stream); break;
@@ -363,16 +417,18 @@ void tune_widths(mlem_batch* B) {
     }
     check_launch("batch width tuning");
     std::sort(t.begin(), t.end());
-    B->wcost[i] = t[t.size() / 2];
+    (W == 1 ? B->wcost1 : B->wcost[k - 1]) = t[t.size() / 2];
+    B->spare.push_back(std::move(blk));
   }
 }
 
 // Block widths for A live frames, widest first. With measured costs: the
 // blocks (each of width <= wmax) that hold A frames at the least summed
 // block-iteration cost, e.g. one 32-wide and one 8-wide block for 40 frames
-// rather than two 32-wide ones (dynamic programming over the frames
+// rather than two 32-wide ones, or a few single-frame blocks once fewer
+// frames are left than make a block pay (dynamic programming over the frames
 // covered). Without: blocks of the narrowest power of two that holds them
-// (at most wmax).
+// (at most wmax), or one single-frame block per frame when wmax is 1.
 std::vector<int> choose_widths(const mlem_batch* B, int A) {
   std::vector<int> ws;
   if (A <= 0) return ws;
@@ -380,7 +436,7 @@ std::vector<int> choose_widths(const mlem_batch* B, int A) {
     std::vector<double> best((size_t)A + 1, -1.0);
     std::vector<int> pick((size_t)A + 1, 0);
     best[0] = 0.0;
-    for (int a = 1; a <= A; ++a)
+    for (int a = 1; a <= A; ++a) {
       for (int i = 0, W = 4; i < 4 && W <= B->wmax; ++i, W *= 2) {
         const double c = best[(size_t)std::max(0, a - W)] + B->wcost[i];
         if (best[(size_t)a] < 0.0 || c < best[(size_t)a]) {
@@ -388,8 +444,17 @@ std::vector<int> choose_widths(const mlem_batch* B, int A) {
           pick[(size_t)a] = W;
         }
       }
+      if (B->wcost1 > 0.0 && best[(size_t)a - 1] + B->wcost1 < best[(size_t)a]) {
+        best[(size_t)a] = best[(size_t)a - 1] + B->wcost1;
+        pick[(size_t)a] = 1;
+      }
+    }
     for (int a = A; a > 0; a = std::max(0, a - pick[(size_t)a])) ws.push_back(pick[(size_t)a]);
     std::sort(ws.rbegin(), ws.rend());
+    return ws;
+  }
+  if (B->wmax == 1) {
+    ws.assign((size_t)A, 1);
     return ws;
   }
   int W = 4;
@@ -408,6 +473,10 @@ void repack(mlem_batch* B, bool force) {
   const size_t nb = ws.size();
   bool same = !force && nb == B->blocks.size();
   for (size_t i = 0; same && i < nb; ++i) same = B->blocks[i]->W == ws[i];
+  // the single-image kernels have no lane mask: a finished frame's block of
+  // width 1 must go
+  for (auto& b : B->blocks)
+    if (b->W == 1 && b->frames[0] >= 0 && !B->active[b->frames[0]]) same = false;
   // save finished frames still living in a block
   for (auto& b : B->blocks)
     for (int l = 0; l < b->W; ++l) {
@@ -425,7 +494,7 @@ void repack(mlem_batch* B, bool force) {
   }
   drop_graph(B);
   std::vector<std::unique_ptr<RhsBlock>> nbk;
-  for (size_t i = 0; i < nb; ++i) nbk.push_back(new_block(B, ws[i]));
+  for (size_t i = 0; i < nb; ++i) nbk.push_back(take_block(B, ws[i]));
   size_t bi = 0;
   int l = 0;
   for (uint32_t f = 0; f < B->F; ++f) {
@@ -452,6 +521,7 @@ void repack(mlem_batch* B, bool force) {
   }
   check_launch("repack");
   MLEM_HIP(hipStreamSynchronize(m.stream));
+  for (auto& b : B->blocks) B->spare.push_back(std::move(b));
   B->blocks = std::move(nbk);
   for (auto& b : B->blocks) upload_mask(B, *b);
 }
@@ -483,9 +553,9 @@ int mlem_batch_create(mlem_matrix* h, uint32_t frames, const int32_t* y, uint32_
       o = *opts;
     else
       mlem_batch_options_default(&o);
-    if (o.block_width != 0 && o.block_width != 4 && o.block_width != 8 && o.block_width != 16 &&
-        o.block_width != 32)
-      raise(MLEM_EINVAL, "block_width must be 0 (autotune), 4, 8, 16 or 32");
+    if (o.block_width != 0 && o.block_width != 1 && o.block_width != 4 && o.block_width != 8 &&
+        o.block_width != 16 && o.block_width != 32)
+      raise(MLEM_EINVAL, "block_width must be 0 (autotune), 1, 4, 8, 16 or 32");
     const bool autotune = o.block_width == 0;
     std::unique_ptr<mlem_batch> B(new mlem_batch());
     B->mat = h;
@@ -505,6 +575,7 @@ int mlem_batch_create(mlem_matrix* h, uint32_t frames, const int32_t* y, uint32_
     for (uint32_t f = 0; f < frames; ++f)
       if (B->budget[f] == 0) B->active[f] = 0;
     B->grid = (uint32_t)std::max(8, 8 * m.cus);
+    B->graph_gen = m.launch_gen;
     // tiles (device-order pointers are on the host after open)
     prepare_tiles(m, m.stream);
     const std::vector<uint32_t> tf = spmm::partition(m.h_rowptr, kTile),
@@ -539,7 +610,7 @@ int mlem_batch_create(mlem_matrix* h, uint32_t frames, const int32_t* y, uint32_
     int A = 0;
     for (uint32_t f = 0; f < frames; ++f) A += B->active[f];
     const std::vector<int> ws = choose_widths(B.get(), A);
-    for (int W : ws) B->blocks.push_back(new_block(B.get(), W));
+    for (int W : ws) B->blocks.push_back(take_block(B.get(), W));
     size_t bi = 0;
     int l = 0;
     for (uint32_t f = 0; f < frames; ++f) {
@@ -567,6 +638,21 @@ int mlem_batch_run(mlem_batch* B, int max_iterations, int* iterations_run) {
   return guarded([&] {
     if (!B || max_iterations < 0) raise(MLEM_EINVAL, "bad argument");
     DevMatrix& m = B->mat->m;
+    if (B->graph_gen != m.launch_gen) {
+      // the matrix's launch configuration changed since the last run: the
+      // single-image kernels of width-1 blocks now use other tile buffers
+      // and perhaps a larger statistics grid
+      drop_graph(B);
+      MLEM_HIP(hipStreamSynchronize(m.stream));
+      const unsigned need = stats_max_blocks(m);
+      for (auto& b : B->blocks)
+        if (b->W == 1 && need > b->pmax) {
+          dfree(b->partials);
+          MLEM_HIP(hipMalloc(&b->partials, sizeof(double2) * (size_t)need));
+          b->pmax = need;
+        }
+      B->graph_gen = m.launch_gen;
+    }
     int it = 0;
     for (; it < max_iterations && !B->blocks.empty(); ++it) {
       if (B->use_graph) {
@@ -589,12 +675,17 @@ int mlem_batch_run(mlem_batch* B, int max_iterations, int* iterations_run) {
       }
       // per-frame statistics of this iteration
       bool changed = false;
-      for (auto& b : B->blocks) {
-        std::vector<double2> r(b->W);
-        const unsigned slot = b->iters % kRing;
-        MLEM_HIP(hipMemcpyAsync(r.data(), b->results + (size_t)slot * b->W,
-                                sizeof(double2) * b->W, hipMemcpyDeviceToHost, m.stream));
-        MLEM_HIP(hipStreamSynchronize(m.stream));
+      std::vector<std::vector<double2>> res(B->blocks.size());
+      for (size_t i = 0; i < B->blocks.size(); ++i) {
+        const RhsBlock& b = *B->blocks[i];
+        res[i].resize((size_t)b.W);
+        MLEM_HIP(hipMemcpyAsync(res[i].data(), b.results + (size_t)(b.iters % kRing) * b.W,
+                                sizeof(double2) * b.W, hipMemcpyDeviceToHost, m.stream));
+      }
+      MLEM_HIP(hipStreamSynchronize(m.stream));
+      for (size_t i = 0; i < B->blocks.size(); ++i) {
+        auto& b = B->blocks[i];
+        const std::vector<double2>& r = res[i];
         b->iters++;
         for (int l = 0; l < b->W; ++l) {
           const int f = b->frames[l];
@@ -664,6 +755,13 @@ int mlem_batch_tuning(const mlem_batch* B, double block_ms[4]) {
   return guarded([&] {
     if (!B || !block_ms) raise(MLEM_EINVAL, "null argument");
     for (int i = 0; i < 4; ++i) block_ms[i] = B->wcost[i];
+  });
+}
+
+int mlem_batch_single_ms(const mlem_batch* B, double* ms) {
+  return guarded([&] {
+    if (!B || !ms) raise(MLEM_EINVAL, "null argument");
+    *ms = B->wcost1;
   });
 }
 

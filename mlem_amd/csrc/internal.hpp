@@ -179,7 +179,7 @@ struct DevMatrix {
 // per-iteration results indexed by a device-side iteration counter.
 struct StatsState {
   double2* partials = nullptr;  // [max blocks]
-  unsigned* counter = nullptr;  // [2]: blocks done, next result slot
+  unsigned* counter = nullptr;  // [2]: [1] = next result slot ([0] unused)
   double2* results = nullptr;   // [ring]
   unsigned ring = 0;
   unsigned max_blocks = 0;

@@ -114,15 +114,15 @@ __device__ __forceinline__ float seg_dot(const uint2* __restrict__ a, const floa
   }
 }
 
-// Deterministic FP64 block + grid reduction of (ll, tot). Every block writes
-// its partial; the block that finishes last sums all partials in a fixed
-// order and appends the result to the ring.
+// Deterministic FP64 reduction of (ll, tot), first stage: every workgroup
+// reduces its lanes in a fixed order and stores its partial. The second
+// stage, k_stats_reduce, runs as the next kernel in the stream; the kernel
+// boundary orders the partials before it, so the projection kernels need no
+// device-scope fence or atomic of their own (a fence per workgroup writes
+// back and invalidates the L2 the gathers depend on).
 template <int NB = kBlock>
-__device__ void stats_finish(double ll, double tot, double2* __restrict__ partials,
-                             unsigned* __restrict__ counter, double2* __restrict__ results,
-                             unsigned ring) {
+__device__ void stats_partial(double ll, double tot, double2* __restrict__ partials) {
   __shared__ double s_ll[NB / kWave], s_tot[NB / kWave];
-  __shared__ int s_last;
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) {
     ll += __shfl_xor(ll, off);
@@ -140,26 +140,29 @@ __device__ void stats_finish(double ll, double tot, double2* __restrict__ partia
       bl += s_ll[w];
       bt += s_tot[w];
     }
-    __hip_atomic_store(&partials[blockIdx.x].x, bl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&partials[blockIdx.x].y, bt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    const unsigned done = atomicAdd(&counter[0], 1u);
-    s_last = (done == gridDim.x - 1);
+    partials[blockIdx.x] = make_double2(bl, bt);
   }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
+}
+
+// Second stage, one workgroup: sum the n partials in a fixed order and
+// append (ll, tot) to the ring at slot counter[1] (then advance it).
+__global__ __launch_bounds__(kBlock) void k_stats_reduce(const double2* __restrict__ partials,
+                                                         unsigned n, unsigned* __restrict__ counter,
+                                                         double2* __restrict__ results,
+                                                         unsigned ring) {
+  __shared__ double s_ll[kBlock / kWave], s_tot[kBlock / kWave];
   double pl = 0.0, pt = 0.0;
-  for (unsigned i = threadIdx.x; i < gridDim.x; i += NB) {
-    pl += __hip_atomic_load(&partials[i].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    pt += __hip_atomic_load(&partials[i].y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (unsigned i = threadIdx.x; i < n; i += kBlock) {
+    const double2 p = partials[i];
+    pl += p.x;
+    pt += p.y;
   }
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) {
     pl += __shfl_xor(pl, off);
     pt += __shfl_xor(pt, off);
   }
-  __syncthreads();
+  const int wave = threadIdx.x / kWave, wl = threadIdx.x % kWave;
   if (wl == 0) {
     s_ll[wave] = pl;
     s_tot[wave] = pt;
@@ -167,13 +170,13 @@ __device__ void stats_finish(double ll, double tot, double2* __restrict__ partia
   __syncthreads();
   if (threadIdx.x == 0) {
     double bl = 0.0, bt = 0.0;
-    for (int w = 0; w < NB / kWave; ++w) {
+    for (int w = 0; w < kBlock / kWave; ++w) {
       bl += s_ll[w];
       bt += s_tot[w];
     }
-    const unsigned slot = atomicAdd(&counter[1], 1u);
+    const unsigned slot = counter[1];
     results[slot % ring] = make_double2(bl, bt);
-    atomicExch(&counter[0], 0u);
+    counter[1] = slot + 1;
   }
 }
 
@@ -234,8 +237,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd(
     const uint32_t* __restrict__ rowptr, const uint2* __restrict__ csr,
     const float* __restrict__ x, const int32_t* __restrict__ y, float* __restrict__ out,
     const float* __restrict__ rscale, uint32_t rows, uint32_t chunk,
-    double2* __restrict__ partials, unsigned* __restrict__ counter,
-    double2* __restrict__ results, unsigned ring) {
+    double2* __restrict__ partials) {
   constexpr uint32_t GPB = kBlock / G;
   const uint32_t lane = threadIdx.x % G;
   const uint32_t grp = threadIdx.x / G;
@@ -253,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd(
       epi.finish(row, acc);
     }
   }
-  if constexpr (STATS) stats_finish(epi.ll, epi.tot, partials, counter, results, ring);
+  if constexpr (STATS) stats_partial(epi.ll, epi.tot, partials);
 }
 
 // --------------------------------------------------------------- backward --
@@ -399,12 +401,11 @@ __global__ __launch_bounds__(kBlock) void k_fwd_tiles(
     const uint32_t* __restrict__ rowptr, const uint2* __restrict__ csr,
     const uint2* __restrict__ tiles, uint32_t ntiles, const float* __restrict__ x,
     const int32_t* __restrict__ y, float* __restrict__ out, const float* __restrict__ rscale,
-    double2* __restrict__ partials, unsigned* __restrict__ counter,
-    double2* __restrict__ results, unsigned ring) {
+    double2* __restrict__ partials) {
   const uint32_t tile = logical_block();
   FwdEpilogue<RATIO, STATS> epi{y, out, rscale};
   if (tile < ntiles) tile_spmv<EPT>(rowptr, csr, tiles, tile, x, epi);
-  if constexpr (STATS) stats_finish(epi.ll, epi.tot, partials, counter, results, ring);
+  if constexpr (STATS) stats_partial(epi.ll, epi.tot, partials);
 }
 
 template <int EPT, bool UPDATE>
@@ -747,13 +748,12 @@ __global__ __launch_bounds__(NB) void k_fwd_dict(
     const uint16_t* __restrict__ dlix, const uint32_t* __restrict__ dict,
     const uint32_t* __restrict__ dmeta, const float* __restrict__ x, uint32_t cols,
     const int32_t* __restrict__ y,
-    float* __restrict__ out, const float* __restrict__ rscale, double2* __restrict__ partials,
-    unsigned* __restrict__ counter, double2* __restrict__ results, unsigned ring) {
+    float* __restrict__ out, const float* __restrict__ rscale, double2* __restrict__ partials) {
   const uint32_t tile = logical_block();
   FwdEpilogue<RATIO, STATS> epi{y, out, rscale};
   if (tile < ntiles)
     dict_spmv<NB, EPT, VK, EXACT>(rowptr, csr, tiles, tile, dval, dlix, dict, dmeta, x, cols, epi);
-  if constexpr (STATS) stats_finish<NB>(epi.ll, epi.tot, partials, counter, results, ring);
+  if constexpr (STATS) stats_partial<NB>(epi.ll, epi.tot, partials);
 }
 
 template <int NB, int EPT, int VK, bool EXACT, bool UPDATE>
@@ -1061,6 +1061,13 @@ inline uint32_t blocks_for(uint64_t n) {
   return (uint32_t)std::max<uint64_t>(1, b);  // n < 2^32 + 1: at most 2^24 blocks
 }
 
+// Second stage of a statistics-carrying forward launch of `blocks`
+// workgroups: one workgroup appends the launch's (ll, tot) to the ring.
+void stats_reduce(const StatsState* st, unsigned blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(kBlock), 0, s, st->partials, blocks, st->counter,
+                     st->results, st->ring);
+}
+
 template <int G, bool EXACT>
 void fwd_g(const DevMatrix& m, const float* x, const int32_t* y, float* out, bool ratio,
            const StatsState* st, hipStream_t s) {
@@ -1068,16 +1075,16 @@ void fwd_g(const DevMatrix& m, const float* x, const int32_t* y, float* out, boo
   const Geometry g = geometry(m.rows, G, m.fwd.blocks_per_cu, m.cus, stats);
   if (stats && g.grid > st->max_blocks) raise(MLEM_ESTATE, "statistics buffer too small for launch");
   const dim3 grid(g.grid), block(kBlock);
-  if (stats)
+  if (stats) {
     hipLaunchKernelGGL((k_fwd<G, EXACT, true, true>), grid, block, 0, s, m.rowptr, m.csr, x, y, out,
-                       m.rscale, m.rows, g.chunk, st->partials, st->counter, st->results,
-                       st->ring);
-  else if (ratio)
+                       m.rscale, m.rows, g.chunk, st->partials);
+    stats_reduce(st, grid.x, s);
+  } else if (ratio)
     hipLaunchKernelGGL((k_fwd<G, EXACT, true, false>), grid, block, 0, s, m.rowptr, m.csr, x, y,
-                       out, m.rscale, m.rows, g.chunk, nullptr, nullptr, nullptr, 0u);
+                       out, m.rscale, m.rows, g.chunk, nullptr);
   else
     hipLaunchKernelGGL((k_fwd<G, EXACT, false, false>), grid, block, 0, s, m.rowptr, m.csr, x, y,
-                       out, m.rscale, m.rows, g.chunk, nullptr, nullptr, nullptr, 0u);
+                       out, m.rscale, m.rows, g.chunk, nullptr);
 }
 
 template <int G, bool EXACT>
@@ -1104,16 +1111,16 @@ void fwd_block_g(const uint32_t* rowptr, const uint2* csr, uint32_t rows, const 
   const Geometry g = geometry(rows, G, 0, cus, stats);
   if (stats && g.grid > st->max_blocks) raise(MLEM_ESTATE, "statistics buffer too small for launch");
   const dim3 grid(g.grid), block(kBlock);
-  if (stats)
+  if (stats) {
     hipLaunchKernelGGL((k_fwd<G, EXACT, true, true>), grid, block, 0, s, rowptr, csr, x, y, out,
-                       (const float*)nullptr, rows, g.chunk, st->partials, st->counter,
-                       st->results, st->ring);
-  else if (ratio)
+                       (const float*)nullptr, rows, g.chunk, st->partials);
+    stats_reduce(st, grid.x, s);
+  } else if (ratio)
     hipLaunchKernelGGL((k_fwd<G, EXACT, true, false>), grid, block, 0, s, rowptr, csr, x, y, out,
-                       (const float*)nullptr, rows, g.chunk, nullptr, nullptr, nullptr, 0u);
+                       (const float*)nullptr, rows, g.chunk, nullptr);
   else
     hipLaunchKernelGGL((k_fwd<G, EXACT, false, false>), grid, block, 0, s, rowptr, csr, x, y, out,
-                       (const float*)nullptr, rows, g.chunk, nullptr, nullptr, nullptr, 0u);
+                       (const float*)nullptr, rows, g.chunk, nullptr);
 }
 
 template <int G, bool EXACT>
@@ -1140,18 +1147,16 @@ void fwd_t(const DevMatrix& m, const float* x, const int32_t* y, float* out, boo
   const bool stats = ratio && st != nullptr;
   const dim3 grid(tile_grid(m.ntiles_fwd)), block(kBlock);
   if (stats && grid.x > st->max_blocks) raise(MLEM_ESTATE, "statistics buffer too small for launch");
-  if (stats)
+  if (stats) {
     hipLaunchKernelGGL((k_fwd_tiles<EPT, true, true>), grid, block, 0, s, m.rowptr, m.csr,
-                       m.tiles_fwd, m.ntiles_fwd, x, y, out, m.rscale, st->partials, st->counter,
-                       st->results, st->ring);
-  else if (ratio)
+                       m.tiles_fwd, m.ntiles_fwd, x, y, out, m.rscale, st->partials);
+    stats_reduce(st, grid.x, s);
+  } else if (ratio)
     hipLaunchKernelGGL((k_fwd_tiles<EPT, true, false>), grid, block, 0, s, m.rowptr, m.csr,
-                       m.tiles_fwd, m.ntiles_fwd, x, y, out, m.rscale, nullptr, nullptr, nullptr,
-                       0u);
+                       m.tiles_fwd, m.ntiles_fwd, x, y, out, m.rscale, nullptr);
   else
     hipLaunchKernelGGL((k_fwd_tiles<EPT, false, false>), grid, block, 0, s, m.rowptr, m.csr,
-                       m.tiles_fwd, m.ntiles_fwd, x, y, out, m.rscale, nullptr, nullptr, nullptr,
-                       0u);
+                       m.tiles_fwd, m.ntiles_fwd, x, y, out, m.rscale, nullptr);
 }
 
 template <int EPT>
@@ -1179,18 +1184,19 @@ void fwd_dvx(const DevMatrix& m, const float* x, const int32_t* y, float* out, b
   const bool stats = ratio && st != nullptr;
   const dim3 grid(tile_grid(D.ntiles)), block(NB);
   if (stats && grid.x > st->max_blocks) raise(MLEM_ESTATE, "statistics buffer too small for launch");
-  if (stats)
+  if (stats) {
     hipLaunchKernelGGL((k_fwd_dict<NB, EPT, VK, EXACT, true, true>), grid, block, 0, s, m.rowptr,
                        m.csr, D.tiles, D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, x, m.cols, y, out,
-                       m.rscale, st->partials, st->counter, st->results, st->ring);
-  else if (ratio)
+                       m.rscale, st->partials);
+    stats_reduce(st, grid.x, s);
+  } else if (ratio)
     hipLaunchKernelGGL((k_fwd_dict<NB, EPT, VK, EXACT, true, false>), grid, block, 0, s, m.rowptr,
                        m.csr, D.tiles, D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, x, m.cols, y, out,
-                       m.rscale, nullptr, nullptr, nullptr, 0u);
+                       m.rscale, nullptr);
   else
     hipLaunchKernelGGL((k_fwd_dict<NB, EPT, VK, EXACT, false, false>), grid, block, 0, s, m.rowptr,
                        m.csr, D.tiles, D.ntiles, D.dval, D.dlix, D.dict, D.dmeta, x, m.cols, y, out,
-                       m.rscale, nullptr, nullptr, nullptr, 0u);
+                       m.rscale, nullptr);
 }
 
 // EXACT sums run on the 512-lane dictionary kernels only (kDictWide).

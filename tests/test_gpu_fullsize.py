@@ -128,6 +128,26 @@ def test_pet_batched_frames_bitwise(pet):
     B.close()
 
 
+def test_pet_batched_autotuned_lone_frames(pet):
+    # at this size a block-iteration costs several lone-frame iterations, so
+    # an autotuned batch of three frames runs each alone on the single-image
+    # EXACT kernels; every frame stays bit-identical to the sequential driver
+    p, A = pet
+    m = p.matrix
+    Y = np.stack([p.y, (p.y // 2).astype(np.int32), (p.y // 5).astype(np.int32)])
+    budgets = np.array([3, 2, 4], np.int32)
+    B = M.Batch(A, Y, budgets, block_width=0)
+    ms = B.tuning()
+    assert 3 * ms[1] < ms[4]
+    prog = B.progress()
+    assert prog["block_width"] == 1 and prog["blocks"] == 3
+    assert B.run(10) == 4
+    for f in range(3):
+        xo = orc.mlem(m.rowidx, m.elements, m.cols, Y[f], int(budgets[f]))[0]
+        assert np.array_equal(B.image(f), xo), f
+    B.close()
+
+
 @pytest.fixture(scope="module")
 def large():
     p = synth.make_problem("large")
