@@ -463,28 +463,30 @@ std::vector<int> choose_widths(const mlem_batch* B, int A) {
   return ws;
 }
 
-// Place the still-active frames into blocks of the chosen widths (frames in
-// index order, blocks widest first); finished frames' images move to final_x.
-void repack(mlem_batch* B, bool force) {
+// Bring the blocks to the widths chosen for the still-active frames;
+// finished frames' images move to final_x. Blocks of a width the new layout
+// keeps stay where they are with their frames in their lanes (those holding
+// the most live frames first); only the frames of dropped blocks move, into
+// free lanes of the kept or new blocks.
+void repack(mlem_batch* B) {
   DevMatrix& m = B->mat->m;
   int A = 0;
   for (uint32_t f = 0; f < B->F; ++f) A += B->active[f] ? 1 : 0;
   const std::vector<int> ws = choose_widths(B, A);
-  const size_t nb = ws.size();
-  bool same = !force && nb == B->blocks.size();
-  for (size_t i = 0; same && i < nb; ++i) same = B->blocks[i]->W == ws[i];
+  bool same = ws.size() == B->blocks.size();
+  for (size_t i = 0; same && i < ws.size(); ++i) same = B->blocks[i]->W == ws[i];
   // the single-image kernels have no lane mask: a finished frame's block of
   // width 1 must go
   for (auto& b : B->blocks)
     if (b->W == 1 && b->frames[0] >= 0 && !B->active[b->frames[0]]) same = false;
-  // save finished frames still living in a block
+  // save finished frames still living in a block; their lanes become idle
   for (auto& b : B->blocks)
     for (int l = 0; l < b->W; ++l) {
       const int f = b->frames[l];
       if (f >= 0 && !B->active[f]) {
         hipLaunchKernelGGL(k_lane_out, dim3(blocks_for(m.cols)), dim3(kBlock), 0, m.stream, b->X,
                            b->W, l, B->final_x + (size_t)f * m.cols, m.colmap, m.cols);
-        if (same) b->frames[l] = -1;
+        b->frames[l] = -1;
       }
     }
   check_launch("lane copy");
@@ -493,35 +495,58 @@ void repack(mlem_batch* B, bool force) {
     return;
   }
   drop_graph(B);
-  std::vector<std::unique_ptr<RhsBlock>> nbk;
-  for (size_t i = 0; i < nb; ++i) nbk.push_back(take_block(B, ws[i]));
+  auto live = [](const RhsBlock& b) {
+    int n = 0;
+    for (int f : b.frames) n += f >= 0 ? 1 : 0;
+    return n;
+  };
+  std::vector<std::unique_ptr<RhsBlock>> old = std::move(B->blocks), nbk;
+  B->blocks.clear();
+  for (int W : ws) {
+    int pick = -1;
+    for (size_t j = 0; j < old.size(); ++j)
+      if (old[j] && old[j]->W == W && (pick < 0 || live(*old[j]) > live(*old[(size_t)pick])))
+        pick = (int)j;
+    if (pick >= 0)
+      nbk.push_back(std::move(old[(size_t)pick]));
+    else
+      nbk.push_back(take_block(B, W));
+  }
+  // frames of the dropped blocks into free lanes (blocks in width order)
   size_t bi = 0;
   int l = 0;
-  for (uint32_t f = 0; f < B->F; ++f) {
-    if (!B->active[f]) continue;
-    if (l == nbk[bi]->W) {
-      ++bi;
-      l = 0;
-    }
-    RhsBlock& d = *nbk[bi];
-    d.frames[l] = (int)f;
-    // find the frame's current lane (if any)
-    bool moved = false;
-    for (auto& b : B->blocks)
-      for (int ls = 0; ls < b->W && !moved; ++ls)
-        if (b->frames[ls] == (int)f) {
-          hipLaunchKernelGGL(k_lane_move, dim3(blocks_for(m.cols)), dim3(kBlock), 0, m.stream,
-                             b->X, b->W, ls, d.X, d.W, l, m.cols);
-          hipLaunchKernelGGL(k_lane_move_i32, dim3(blocks_for(m.rows)), dim3(kBlock), 0,
-                             m.stream, b->Y, b->W, ls, d.Y, d.W, l, m.rows);
-          moved = true;
+  for (auto& src : old) {
+    if (!src) continue;
+    for (int ls = 0; ls < src->W; ++ls) {
+      const int f = src->frames[ls];
+      if (f < 0) continue;
+      while (bi < nbk.size() && (l == nbk[bi]->W || nbk[bi]->frames[l] >= 0)) {
+        if (++l >= nbk[bi]->W) {
+          ++bi;
+          l = 0;
         }
-    if (!moved) raise(MLEM_ESTATE, "batched frame lost during repacking");
-    ++l;
+      }
+      if (bi == nbk.size()) raise(MLEM_ESTATE, "batched frame lost during repacking");
+      RhsBlock& d = *nbk[bi];
+      hipLaunchKernelGGL(k_lane_move, dim3(blocks_for(m.cols)), dim3(kBlock), 0, m.stream, src->X,
+                         src->W, ls, d.X, d.W, l, m.cols);
+      hipLaunchKernelGGL(k_lane_move_i32, dim3(blocks_for(m.rows)), dim3(kBlock), 0, m.stream,
+                         src->Y, src->W, ls, d.Y, d.W, l, m.rows);
+      d.frames[l] = f;
+    }
   }
   check_launch("repack");
   MLEM_HIP(hipStreamSynchronize(m.stream));
-  for (auto& b : B->blocks) B->spare.push_back(std::move(b));
+  for (auto& b : old)
+    if (b) B->spare.push_back(std::move(b));
+  // a single-frame block left without a frame would still run its kernels
+  for (auto& b : nbk)
+    if (b->W == 1 && b->frames[0] < 0) B->spare.push_back(std::move(b));
+  nbk.erase(std::remove(nbk.begin(), nbk.end(), nullptr), nbk.end());
+  std::stable_sort(nbk.begin(), nbk.end(),
+                   [](const std::unique_ptr<RhsBlock>& x, const std::unique_ptr<RhsBlock>& y) {
+                     return x->W > y->W;
+                   });
   B->blocks = std::move(nbk);
   for (auto& b : B->blocks) upload_mask(B, *b);
 }
@@ -705,7 +730,7 @@ int mlem_batch_run(mlem_batch* B, int max_iterations, int* iterations_run) {
           }
         }
       }
-      if (changed) repack(B, false);
+      if (changed) repack(B);
     }
     if (iterations_run) *iterations_run = it;
   });
