@@ -68,23 +68,34 @@ __device__ void batch_stats_partial(double ll, double tot, double2* __restrict__
   }
 }
 
-// Second stage, one workgroup: per frame, the sum of the n workgroup
-// partials in a fixed order, appended to the ring at slot counter[1].
+// Second stage, one 1024-lane workgroup: per frame, the sum of the n
+// workgroup partials in a fixed order (lane group g takes workgroups g,
+// g + G, ... into two alternating accumulators), appended to the ring at
+// slot counter[1].
+constexpr int kReduceLanes = 1024;
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_batch_stats_reduce(
+__global__ __launch_bounds__(kReduceLanes) void k_batch_stats_reduce(
     const double2* __restrict__ partials, unsigned n, unsigned* __restrict__ counter,
     double2* __restrict__ results, unsigned ring) {
-  constexpr int G = kBlock / W;
-  __shared__ double s_ll[kBlock], s_tot[kBlock];
+  constexpr int G = kReduceLanes / W;
+  __shared__ double s_ll[kReduceLanes], s_tot[kReduceLanes];
   const uint32_t t = threadIdx.x, f = t % W, g = t / W;
-  double a = 0.0, b = 0.0;
-  for (uint32_t blk = g; blk < n; blk += G) {
-    const double2 p = partials[(size_t)blk * W + f];
-    a += p.x;
-    b += p.y;
+  double a0 = 0.0, b0 = 0.0, a1 = 0.0, b1 = 0.0;
+  uint32_t blk = g;
+  for (; blk + G < n; blk += 2 * G) {
+    const double2 p = partials[(size_t)blk * W + f], q = partials[(size_t)(blk + G) * W + f];
+    a0 += p.x;
+    b0 += p.y;
+    a1 += q.x;
+    b1 += q.y;
   }
-  s_ll[t] = a;
-  s_tot[t] = b;
+  if (blk < n) {
+    const double2 p = partials[(size_t)blk * W + f];
+    a0 += p.x;
+    b0 += p.y;
+  }
+  s_ll[t] = a0 + a1;
+  s_tot[t] = b0 + b1;
   __syncthreads();
   const unsigned slot = counter[1];
   if (t < (uint32_t)W) {
@@ -340,8 +351,8 @@ void launch_block(mlem_batch* B, RhsBlock& b, hipStream_t s) {
   const DevMatrix& m = B->mat->m;
   hipLaunchKernelGGL((k_fwd_batch<W>), dim3(B->grid), dim3(kBlock), 0, s, m.rowptr, m.csr,
                      B->tiles_f, B->nt_f, b.X, b.Y, b.R, b.partials);
-  hipLaunchKernelGGL((k_batch_stats_reduce<W>), dim3(1), dim3(kBlock), 0, s, b.partials, B->grid,
-                     b.counter, b.results, kRing);
+  hipLaunchKernelGGL((k_batch_stats_reduce<W>), dim3(1), dim3(kReduceLanes), 0, s, b.partials,
+                     B->grid, b.counter, b.results, kRing);
   hipLaunchKernelGGL((k_bwd_batch<W>), dim3(B->grid), dim3(kBlock), 0, s, m.colptr, m.csc,
                      B->tiles_b, B->nt_b, b.R, m.norm, b.X, b.mask);
 }

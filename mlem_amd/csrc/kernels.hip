@@ -144,19 +144,30 @@ __device__ void stats_partial(double ll, double tot, double2* __restrict__ parti
   }
 }
 
-// Second stage, one workgroup: sum the n partials in a fixed order and
-// append (ll, tot) to the ring at slot counter[1] (then advance it).
-__global__ __launch_bounds__(kBlock) void k_stats_reduce(const double2* __restrict__ partials,
-                                                         unsigned n, unsigned* __restrict__ counter,
-                                                         double2* __restrict__ results,
-                                                         unsigned ring) {
-  __shared__ double s_ll[kBlock / kWave], s_tot[kBlock / kWave];
-  double pl = 0.0, pt = 0.0;
-  for (unsigned i = threadIdx.x; i < n; i += kBlock) {
-    const double2 p = partials[i];
-    pl += p.x;
-    pt += p.y;
+// Second stage, one 1024-lane workgroup: sum the n partials in a fixed order
+// (lane t takes partials t, t + 1024, ... into two alternating accumulators,
+// so two loads are in flight) and append (ll, tot) to the ring at slot
+// counter[1] (then advance it).
+constexpr int kReduceLanes = 1024;
+__global__ __launch_bounds__(kReduceLanes) void k_stats_reduce(
+    const double2* __restrict__ partials, unsigned n, unsigned* __restrict__ counter,
+    double2* __restrict__ results, unsigned ring) {
+  __shared__ double s_ll[kReduceLanes / kWave], s_tot[kReduceLanes / kWave];
+  double l0 = 0.0, t0 = 0.0, l1 = 0.0, t1 = 0.0;
+  unsigned i = threadIdx.x;
+  for (; i + kReduceLanes < n; i += 2 * kReduceLanes) {
+    const double2 p = partials[i], q = partials[i + kReduceLanes];
+    l0 += p.x;
+    t0 += p.y;
+    l1 += q.x;
+    t1 += q.y;
   }
+  if (i < n) {
+    const double2 p = partials[i];
+    l0 += p.x;
+    t0 += p.y;
+  }
+  double pl = l0 + l1, pt = t0 + t1;
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) {
     pl += __shfl_xor(pl, off);
@@ -170,7 +181,7 @@ __global__ __launch_bounds__(kBlock) void k_stats_reduce(const double2* __restri
   __syncthreads();
   if (threadIdx.x == 0) {
     double bl = 0.0, bt = 0.0;
-    for (int w = 0; w < kBlock / kWave; ++w) {
+    for (int w = 0; w < kReduceLanes / kWave; ++w) {
       bl += s_ll[w];
       bt += s_tot[w];
     }
@@ -1064,8 +1075,8 @@ inline uint32_t blocks_for(uint64_t n) {
 // Second stage of a statistics-carrying forward launch of `blocks`
 // workgroups: one workgroup appends the launch's (ll, tot) to the ring.
 void stats_reduce(const StatsState* st, unsigned blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(kBlock), 0, s, st->partials, blocks, st->counter,
-                     st->results, st->ring);
+  hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(kReduceLanes), 0, s, st->partials, blocks,
+                     st->counter, st->results, st->ring);
 }
 
 template <int G, bool EXACT>
