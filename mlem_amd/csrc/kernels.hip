@@ -118,8 +118,9 @@ __device__ __forceinline__ float seg_dot(const uint2* __restrict__ a, const floa
 // reduces its lanes in a fixed order and stores its partial. The second
 // stage, k_stats_reduce, runs as the next kernel in the stream; the kernel
 // boundary orders the partials before it, so the projection kernels need no
-// device-scope fence or atomic of their own (a fence per workgroup writes
-// back and invalidates the L2 the gathers depend on).
+// device-scope fence or atomic of their own (with a fence and a counter
+// atomic in every workgroup, the EXACT forward with statistics took twice
+// as long as without statistics).
 template <int NB = kBlock>
 __device__ void stats_partial(double ll, double tot, double2* __restrict__ partials) {
   __shared__ double s_ll[NB / kWave], s_tot[NB / kWave];
