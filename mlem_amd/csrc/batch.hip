@@ -110,19 +110,19 @@ __global__ __launch_bounds__(kReduceLanes) void k_batch_stats_reduce(
   if (t == 0) counter[1] = slot + 1;
 }
 
-// Forward epilogue: ratio r = (f != 0) ? y/f : 0 and the frame's FP64
-// statistics.
+// Forward epilogue: ratio r = (f != 0) ? y/f : 0, and f kept for the
+// statistics kernel. The FP64 log-likelihood is not computed here: its
+// registers would cap the projection at 5 waves per SIMD instead of 7.
 template <int W>
 struct FwdRatio {
   const int32_t* __restrict__ Y;
   float* __restrict__ R;
-  double ll = 0.0, tot = 0.0;
+  float* __restrict__ Fp;
   __device__ void operator()(uint32_t row, uint32_t f, float F) {
     const size_t i = (size_t)row * W + f;
     const int32_t yi = Y[i];
     R[i] = (F != 0.f) ? (float)yi / F : 0.f;
-    tot += (double)F;
-    if (F > 0.f) ll += (yi != 0 ? (double)yi * log((double)F) : 0.0) - (double)F;
+    Fp[i] = F;
   }
 };
 
@@ -141,16 +141,37 @@ struct BwdUpdate {
   }
 };
 
-// Forward projection + ratio (+ per-frame statistics) for one RHS block.
+// Forward projection + ratio for one RHS block (f kept in Fp).
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_fwd_batch(
     const uint32_t* __restrict__ rowptr, const uint2* __restrict__ csr,
     const uint2* __restrict__ tiles, uint32_t ntiles,
     const float* __restrict__ X, const int32_t* __restrict__ Y, float* __restrict__ R,
-    double2* __restrict__ partials) {
-  FwdRatio<W> epi{Y, R};
+    float* __restrict__ Fp) {
+  FwdRatio<W> epi{Y, R, Fp};
   spmm::walk<W>(rowptr, csr, tiles, ntiles, X, epi);
-  batch_stats_partial<W>(epi.ll, epi.tot, partials);
+}
+
+// Per-frame FP64 statistics of a block's forward projection: lane f of row
+// group g sums, over rows g, g + groups, ..., tot += f_i and, where f_i > 0,
+// ll += y_i log f_i - f_i; one partial per workgroup and frame.
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_batch_loglik(const float* __restrict__ Fp,
+                                                         const int32_t* __restrict__ Y,
+                                                         uint32_t rows,
+                                                         double2* __restrict__ partials) {
+  constexpr uint32_t G = kBlock / W;
+  const uint32_t f = threadIdx.x % W, g = threadIdx.x / W;
+  const uint32_t stride = gridDim.x * G;
+  double ll = 0.0, tot = 0.0;
+  for (uint32_t row = blockIdx.x * G + g; row < rows; row += stride) {
+    const size_t i = (size_t)row * W + f;
+    const float F = Fp[i];
+    const int32_t yi = Y[i];
+    tot += (double)F;
+    if (F > 0.f) ll += (yi != 0 ? (double)yi * log((double)F) : 0.0) - (double)F;
+  }
+  batch_stats_partial<W>(ll, tot, partials);
 }
 
 // Backprojection + update for one RHS block over the CSC column tiles.
@@ -226,6 +247,7 @@ struct RhsBlock {
   float* X = nullptr;       // [cols][W], device voxel order
   int32_t* Y = nullptr;     // [rows][W], device row order
   float* R = nullptr;       // [rows][W]
+  float* F = nullptr;       // [rows][W] forward projections (statistics; W > 1)
   uint32_t* mask = nullptr; // bit f set: lane f still iterating
   double2* partials = nullptr;
   unsigned pmax = 0;           // partials held, in workgroups (width 1)
@@ -236,6 +258,7 @@ struct RhsBlock {
     dfree(X);
     dfree(Y);
     dfree(R);
+    dfree(F);
     dfree(mask);
     dfree(partials);
     dfree(counter);
@@ -300,6 +323,7 @@ std::unique_ptr<RhsBlock> new_block(mlem_batch* B, int W) {
   MLEM_HIP(hipMalloc(&b->X, 4ull * m.cols * W));
   MLEM_HIP(hipMalloc(&b->Y, 4ull * m.rows * W));
   MLEM_HIP(hipMalloc(&b->R, 4ull * m.rows * W));
+  if (W > 1) MLEM_HIP(hipMalloc(&b->F, 4ull * m.rows * W));
   MLEM_HIP(hipMalloc(&b->mask, 4));
   // width 1 runs the single-image forward kernel, whose grid follows the
   // matrix's launch configuration
@@ -350,7 +374,9 @@ template <int W>
 void launch_block(mlem_batch* B, RhsBlock& b, hipStream_t s) {
   const DevMatrix& m = B->mat->m;
   hipLaunchKernelGGL((k_fwd_batch<W>), dim3(B->grid), dim3(kBlock), 0, s, m.rowptr, m.csr,
-                     B->tiles_f, B->nt_f, b.X, b.Y, b.R, b.partials);
+                     B->tiles_f, B->nt_f, b.X, b.Y, b.R, b.F);
+  hipLaunchKernelGGL((k_batch_loglik<W>), dim3(B->grid), dim3(kBlock), 0, s, b.F, b.Y, m.rows,
+                     b.partials);
   hipLaunchKernelGGL((k_batch_stats_reduce<W>), dim3(1), dim3(kReduceLanes), 0, s, b.partials,
                      B->grid, b.counter, b.results, kRing);
   hipLaunchKernelGGL((k_bwd_batch<W>), dim3(B->grid), dim3(kBlock), 0, s, m.colptr, m.csc,

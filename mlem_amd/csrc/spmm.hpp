@@ -30,13 +30,12 @@ constexpr uint32_t kTile = 2048;  // records staged per tile (16 KB of LDS)
 constexpr int kUnroll = MLEM_SPMM_UNROLL;  // gathers in flight per lane
 
 // Sequential fp32 dot product of one segment's records with vector f of an
-// interleaved block. The gathers of kUnroll consecutive records are issued
+// interleaved block. The gathers of U consecutive records are issued
 // together (the segment's last group with clamped indices), the products are
 // then added one by one in element order.
-template <int W>
+template <int W, int U = kUnroll>
 __device__ __forceinline__ float seg_dot(const uint2* rec, uint32_t b, uint32_t e,
                                          const float* __restrict__ v, uint32_t f) {
-  constexpr int U = kUnroll;
   float acc = 0.f;
   for (uint32_t k = b; k < e; k += U) {
     float a[U], g[U];
@@ -56,7 +55,8 @@ __device__ __forceinline__ float seg_dot(const uint2* rec, uint32_t b, uint32_t 
 // Walk the tiles [0, ntiles) of a segment partition (tiles[t] = {first
 // segment, first element}) and call epi(segment, f, dot) for every segment
 // and vector. Tiles hold at most 256 segments and kTile records, or one
-// longer segment (read from HBM). Workgroup b runs on XCD b % 8 (the grid is
+// longer segment (read from HBM, 4 records at a time: its 64-bit record
+// addresses would otherwise set the kernel's register count). Workgroup b runs on XCD b % 8 (the grid is
 // a multiple of 8): XCD x walks the x-th contiguous eighth of the tiles, so
 // neighbouring segments (rows of one sinogram region, voxels of one brick
 // range) share that XCD's L2.
@@ -83,7 +83,7 @@ __device__ __forceinline__ void walk(const uint32_t* __restrict__ ptr,
     __syncthreads();
     for (uint32_t sg = grp; sg < nseg; sg += G) {
       const float d = staged ? seg_dot<W>(rec, S[sg], S[sg + 1], v, f)
-                             : seg_dot<W>(el + e0, S[sg], S[sg + 1], v, f);
+                             : seg_dot<W, 4>(el + e0, S[sg], S[sg + 1], v, f);
       epi(s0 + sg, f, d);
     }
   }
