@@ -12,9 +12,9 @@
 // multiply and add: per frame, the result is bit-identical to the reference's
 // sequential driver (mlem_seq/seqcsr4mlem.cpp:69-134).
 //
-// Per frame and iteration the forward pass also produces the FP64 Poisson
-// log-likelihood sum_{f_i != 0} (y_i log f_i - f_i) and sum_i f_i with a
-// deterministic two-level reduction; the host stops a frame when its budget
+// Per frame and iteration, kernels after the forward pass produce the FP64
+// Poisson log-likelihood sum_{f_i != 0} (y_i log f_i - f_i) and sum_i f_i with
+// a deterministic two-stage reduction; the host stops a frame when its budget
 // is spent or the relative log-likelihood change drops below the tolerance,
 // freezes it, and repacks the remaining frames into fewer / narrower blocks.
 // An autotuned batch may also give a frame a block of its own (width 1): it
@@ -49,7 +49,7 @@ void check_launch(const char* what) {
 // Deterministic per-frame FP64 reduction of (ll, tot), first stage: every
 // workgroup sums its lane groups in a fixed order and stores one partial per
 // frame. The second stage (k_batch_stats_reduce) is the next kernel in the
-// stream, so no device-scope fence or atomic runs in the projection kernel.
+// stream, so no workgroup needs a device-scope fence or atomic.
 template <int W>
 __device__ void batch_stats_partial(double ll, double tot, double2* __restrict__ partials) {
   constexpr int G = kBlock / W;
