@@ -479,6 +479,20 @@ void collect_stats(mlem_recon* R, int first, int count) {
 
 void run_iterations(mlem_recon* R, int iterations) {
   DevMatrix& m = R->mat->m;
+  if (R->want_stats && stats_max_blocks(m) > R->st.max_blocks) {
+    // a launch change since creation needs more workgroup partials; the
+    // captured graph holds the old buffer
+    if (R->exec) {
+      (void)hipGraphExecDestroy(R->exec);
+      (void)hipGraphDestroy(R->graph);
+      R->exec = nullptr;
+      R->graph = nullptr;
+    }
+    MLEM_HIP(hipStreamSynchronize(m.stream));
+    dfree(R->st.partials);
+    R->st.max_blocks = stats_max_blocks(m);
+    MLEM_HIP(hipMalloc(&R->st.partials, sizeof(double2) * R->st.max_blocks));
+  }
   int left = iterations;
   while (left > 0) {
     const int n =

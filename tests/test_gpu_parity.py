@@ -162,6 +162,36 @@ def test_launch_change_between_runs_recaptures(tiny):
     A.set_launch(("dict512", 32), ("dict512", 32))
 
 
+def test_recon_statistics_grid_grows_with_launch_change(tiny):
+    # a reconstruction created with statistics under a launch of few
+    # workgroups keeps running after a change to 64 lanes per row (a grid of
+    # 1024 workgroups on this matrix needs more statistics partials)
+    import ctypes as C
+    from mlem_amd._native import lib
+    p, A = tiny
+    m = p.matrix
+    A.set_launch(("dict512", 32), ("dict512", 32))
+    ll = np.zeros(1, np.float64)
+    o = M._options("fast", False, use_graph=True, loglik=ll)
+    y = np.ascontiguousarray(p.y, np.int32)
+    h = C.c_void_p()
+    M.check(lib().mlem_recon_create(A.handle, y.ctypes.data_as(C.c_void_p), y.size, None,
+                                    A.columns(), C.byref(o), C.byref(h)))
+    try:
+        M.check(lib().mlem_recon_run(h, 2))
+        A.set_launch(("groups", 64), ("groups", 64))
+        M.check(lib().mlem_recon_run(h, 2))
+        A.set_launch(("dict512", 32), ("dict512", 32))
+        M.check(lib().mlem_recon_run(h, 2))
+        x = np.empty(A.columns(), np.float32)
+        M.check(lib().mlem_recon_image(h, x.ctypes.data_as(C.c_void_p), x.size))
+    finally:
+        lib().mlem_recon_destroy(h)
+        A.set_launch(("dict512", 32), ("dict512", 32))
+    xo = orc.mlem(m.rowidx, m.elements, m.cols, p.y, 6)[0]
+    assert rel_linf(x, xo) < 1e-5
+
+
 def test_stats_loglik_and_counts(tiny):
     p, A = tiny
     m = p.matrix
