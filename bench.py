@@ -151,12 +151,13 @@ def cpu_baseline(problem, iters: int):
 
 
 def streamed_run(A, prob, block_mib: int, iters: int, plain_value: float,
-                 resident_frac: float = 0.0) -> dict:
+                 resident_frac: float = 0.0, mode: str = "fast") -> dict:
     """The same reconstruction with the matrix kept in pinned host memory and
     streamed to the device in row blocks of at most `block_mib` MiB per part
     (out-of-core path): iterations/s, host-to-device bytes per iteration
-    (every block's CSR part for the forward, CSC part for the backward) and the
-    PCIe rate they imply, plus the image's distance to the in-core run."""
+    (every block's CSR part for the forward, CSC part for the backward; in
+    ATOMIC mode the CSR part only, scattered while it is on the device) and the
+    PCIe rate they imply, plus the image's distance to the in-core FAST run."""
     import mlem_amd as M
     m = prob.matrix
     t0 = time.perf_counter()
@@ -166,7 +167,7 @@ def streamed_run(A, prob, block_mib: int, iters: int, plain_value: float,
                                  stream_resident_bytes=resident)
     open_s = time.perf_counter() - t0
     nb, nr = S.stream_blocks(), S.resident_blocks()
-    R = M.Recon(S, prob.y, mode="fast")
+    R = M.Recon(S, prob.y, mode=mode)
     R.run(1)
     M.device_sync()
     t0 = time.perf_counter()
@@ -179,8 +180,12 @@ def streamed_run(A, prob, block_mib: int, iters: int, plain_value: float,
     xh = M.mlem(A, prob.y, None, iters + 1, mode="fast")
     d = np.abs(xs.astype(np.float64) - xh.astype(np.float64))
     floor = 1e-6 * float(np.max(np.abs(xh)))
-    moved = (16 * m.nnz + 4 * (m.rows + nb * (m.cols + 1))) * (nb - nr) / nb
-    return {"block_mib": block_mib, "blocks": nb, "resident_blocks": nr, "iterations": iters,
+    if mode == "atomic":
+        moved = (8 * m.nnz + 4 * (m.rows + nb)) * (nb - nr) / nb
+    else:
+        moved = (16 * m.nnz + 4 * (m.rows + nb * (m.cols + 1))) * (nb - nr) / nb
+    return {"mode": mode, "block_mib": block_mib, "blocks": nb, "resident_blocks": nr,
+            "iterations": iters,
             "ms_per_iteration": 1e3 * wall / iters, "iterations_per_s": iters / wall,
             "h2d_bytes_per_iteration": moved, "h2d_gbs": moved * iters / wall / 1e9,
             "slowdown_vs_hbm": plain_value / (iters / wall), "open_s": open_s,
@@ -373,6 +378,8 @@ def main() -> int:
             streamed = streamed_run(A, prob, args.streamed_block_mib, args.streamed_iters, value)
             streamed["half_resident"] = streamed_run(A, prob, args.streamed_block_mib,
                                                      args.streamed_iters, value, 0.5)
+            streamed["atomic"] = streamed_run(A, prob, args.streamed_block_mib,
+                                              args.streamed_iters, value, 0.0, "atomic")
         except Exception as e:  # reported in the line, never hides the plain result
             streamed = {"error": f"{type(e).__name__}: {e}"}
         print(f"[bench] streamed: {streamed}", file=sys.stderr, flush=True)

@@ -137,6 +137,11 @@ MLEM_API int mlem_open_host(uint32_t rows, uint32_t cols, uint64_t nnz,
  * copy of one block overlapping the kernels of the block before.
  * A column's backprojection continues its running sum from block to block in
  * ascending row order, so EXACT results equal the in-core ones bit for bit.
+ * In a reconstruction in MLEM_MODE_ATOMIC each block's CSR part crosses PCIe
+ * once per iteration: its forward projection and ratio, then the atomic
+ * scatter of its rows, run while it is on the device (the reference's
+ * out-of-core driver copies the parts for its forward loop and again for
+ * its backprojection loop, mlem_cuda/versions/version_8.cu:214-252).
  * Every step function, mode and mlem_reconstruct / mlem_recon_* work on such a
  * handle (iterations run without hipGraph capture; statistics need at most 4096
  * blocks); the launch configuration is fixed (lane groups), and the batched
@@ -252,7 +257,9 @@ MLEM_API int mlem_recon_run(mlem_recon* r, int iterations);
  * per-kernel durations in milliseconds (arrays of length `iterations`, any
  * may be NULL) and the whole span. kernel_fwd: forward+ratio launches;
  * kernel_bwd: backprojection+update launches (ATOMIC mode: scatter and
- * update summed). */
+ * update summed; a streamed matrix in ATOMIC mode scatters each row block
+ * right after its forward projection, so kernel_fwd holds both and
+ * kernel_bwd the update). */
 MLEM_API int mlem_recon_run_timed(mlem_recon* r, int iterations, float* kernel_fwd_ms,
                          float* kernel_bwd_ms, float* span_ms);
 /* Copy the current image to the host. */

@@ -448,8 +448,12 @@ void enqueue_iteration(mlem_recon* R) {
   DevMatrix& m = R->mat->m;
   const StatsState* st = R->want_stats ? &R->st : nullptr;
   if (R->mode == MLEM_MODE_ATOMIC) {
-    launch_fwd(m, MLEM_MODE_FAST, R->x, R->y, R->r, true, st, m.stream);
-    launch_scatter(m, R->r, R->u, m.stream);
+    if (m.streamed) {
+      stream_fwd_scatter(m, R->x, R->y, R->r, R->u, st, m.stream);
+    } else {
+      launch_fwd(m, MLEM_MODE_FAST, R->x, R->y, R->r, true, st, m.stream);
+      launch_scatter(m, R->r, R->u, m.stream);
+    }
     launch_update(m.cols, R->u, m.norm, R->x, true, m.stream);
   } else {
     launch_fwd(m, R->mode, R->x, R->y, R->r, true, st, m.stream);
@@ -1040,9 +1044,14 @@ int mlem_recon_run_timed(mlem_recon* R, int iterations, float* kf, float* kb, fl
     MLEM_HIP(hipEventRecord(ev[0], m.stream));
     for (int i = 0; i < iterations; ++i) {
       if (R->mode == MLEM_MODE_ATOMIC) {
-        launch_fwd(m, MLEM_MODE_FAST, R->x, R->y, R->r, true, nullptr, m.stream);
-        MLEM_HIP(hipEventRecord(ev[2 * i + 1], m.stream));
-        launch_scatter(m, R->r, R->u, m.stream);
+        if (m.streamed) {  // one fused pass: its time counts as the forward's
+          stream_fwd_scatter(m, R->x, R->y, R->r, R->u, nullptr, m.stream);
+          MLEM_HIP(hipEventRecord(ev[2 * i + 1], m.stream));
+        } else {
+          launch_fwd(m, MLEM_MODE_FAST, R->x, R->y, R->r, true, nullptr, m.stream);
+          MLEM_HIP(hipEventRecord(ev[2 * i + 1], m.stream));
+          launch_scatter(m, R->r, R->u, m.stream);
+        }
         launch_update(m.cols, R->u, m.norm, R->x, true, m.stream);
       } else {
         launch_fwd(m, R->mode, R->x, R->y, R->r, true, nullptr, m.stream);

@@ -209,6 +209,9 @@ void launch_fwd_block(const uint32_t* rowptr, const uint2* csr, uint32_t rows, i
                       const StatsState* st, int cus, hipStream_t s);
 void launch_bwd_block_add(const uint32_t* colptr, const uint2* csc, uint32_t cols, int lanes,
                           bool exact, const float* r, float* u, int cus, hipStream_t s);
+// u += (block rows)^T r by atomic scatter over one row block's CSR.
+void launch_scatter_block(const uint32_t* rowptr, const uint2* csr, uint32_t rows, int lanes,
+                          const float* r, float* u, int cus, hipStream_t s);
 // Streamed matrices (stream.cpp): build the blocks from a row index (END
 // offsets) and an element source; the projections over every block.
 using ElementFetch = std::function<void(void* dst, uint64_t first, uint64_t count)>;
@@ -220,6 +223,12 @@ void stream_fwd(const DevMatrix& m, int mode, const float* x, const int32_t* y, 
 void stream_bwd(const DevMatrix& m, int mode, const float* r, const float* norm, float* out,
                 bool update, hipStream_t s);
 void stream_bwd_add(const DevMatrix& m, bool exact, const float* r, float* u, hipStream_t s);
+// One pass over the CSR blocks: per block the forward projection with the
+// fused ratio into r, then u += (block rows)^T r by atomic scatter while the
+// block is on the device (ATOMIC mode: each block crosses PCIe once per
+// iteration instead of twice).
+void stream_fwd_scatter(const DevMatrix& m, const float* x, const int32_t* y, float* r, float* u,
+                        const StatsState* st, hipStream_t s);
 // x *= (n != 0) ? u/n : u; optionally u = 0 afterwards.
 void launch_update(uint32_t cols, const float* u, const float* norm, float* x,
                    bool zero_u, hipStream_t s);

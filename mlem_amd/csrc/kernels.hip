@@ -1144,6 +1144,14 @@ void bwd_block_add_g(const uint32_t* colptr, const uint2* csc, uint32_t cols, co
 }
 
 template <int G>
+void scatter_block_g(const uint32_t* rowptr, const uint2* csr, uint32_t rows, const float* r,
+                     float* u, int cus, hipStream_t s) {
+  const Geometry g = geometry(rows, G, 0, cus, false);
+  hipLaunchKernelGGL((k_scatter<G>), dim3(g.grid), dim3(kBlock), 0, s, rowptr, csr, r, u, rows,
+                     g.chunk);
+}
+
+template <int G>
 void scatter_g(const DevMatrix& m, const float* r, float* u, hipStream_t s) {
   const Geometry g = geometry(m.rows, G, m.scatter.blocks_per_cu, m.cus, false);
   hipLaunchKernelGGL((k_scatter<G>), dim3(g.grid), dim3(kBlock), 0, s, m.rowptr, m.csr, r, u,
@@ -1602,6 +1610,19 @@ void launch_scatter(const DevMatrix& m, const float* r, float* u, hipStream_t s)
     default: raise(MLEM_EINVAL, "unsupported scatter lane count");
   }
   check_launch("scatter launch");
+}
+
+void launch_scatter_block(const uint32_t* rowptr, const uint2* csr, uint32_t rows, int lanes,
+                          const float* r, float* u, int cus, hipStream_t s) {
+  switch (lanes) {
+    case 4: scatter_block_g<4>(rowptr, csr, rows, r, u, cus, s); break;
+    case 8: scatter_block_g<8>(rowptr, csr, rows, r, u, cus, s); break;
+    case 16: scatter_block_g<16>(rowptr, csr, rows, r, u, cus, s); break;
+    case 32: scatter_block_g<32>(rowptr, csr, rows, r, u, cus, s); break;
+    case 64: scatter_block_g<64>(rowptr, csr, rows, r, u, cus, s); break;
+    default: raise(MLEM_EINVAL, "unsupported scatter lane count");
+  }
+  check_launch("streamed scatter launch");
 }
 
 void launch_update(uint32_t cols, const float* u, const float* norm, float* x, bool zero_u,

@@ -248,6 +248,21 @@ void stream_bwd_add(const DevMatrix& m, bool exact, const float* r, float* u, hi
       s);
 }
 
+void stream_fwd_scatter(const DevMatrix& m, const float* x, const int32_t* y, float* r, float* u,
+                        const StatsState* st, hipStream_t s) {
+  const StreamedMatrix& S = *m.streamed;
+  stream_pass(
+      m, false,
+      [&](const StreamBlock& B, char* d) {
+        const uint32_t* rp = reinterpret_cast<const uint32_t*>(d);
+        const uint2* el = reinterpret_cast<const uint2*>(d + align16(4ull * ((uint64_t)B.rows + 1)));
+        launch_fwd_block(rp, el, B.rows, S.lanes_fwd, false, x, y + B.row0, r + B.row0, true, st,
+                         m.cus, s);
+        if (B.rows) launch_scatter_block(rp, el, B.rows, S.lanes_fwd, r + B.row0, u, m.cus, s);
+      },
+      s);
+}
+
 void stream_bwd(const DevMatrix& m, int mode, const float* r, const float* norm, float* out,
                 bool update, hipStream_t s) {
   float* acc = update ? m.streamed->u : out;
