@@ -155,9 +155,10 @@ def streamed_run(A, prob, block_mib: int, iters: int, plain_value: float,
     """The same reconstruction with the matrix kept in pinned host memory and
     streamed to the device in row blocks of at most `block_mib` MiB per part
     (out-of-core path): iterations/s, host-to-device bytes per iteration
-    (every block's CSR part for the forward, CSC part for the backward; in
-    ATOMIC mode the CSR part only, scattered while it is on the device) and the
-    PCIe rate they imply, plus the image's distance to the in-core FAST run."""
+    (every streamed block's CSR part, once: its CSC is built on the device, or
+    in ATOMIC mode its rows are scattered while it is there; the CSC parts too
+    when the device CSC did not fit) and the PCIe rate they imply, plus the
+    image's distance to the in-core FAST run."""
     import mlem_amd as M
     m = prob.matrix
     t0 = time.perf_counter()
@@ -166,7 +167,7 @@ def streamed_run(A, prob, block_mib: int, iters: int, plain_value: float,
                                  stream_block_bytes=block_mib << 20,
                                  stream_resident_bytes=resident)
     open_s = time.perf_counter() - t0
-    nb, nr = S.stream_blocks(), S.resident_blocks()
+    nb, nr, passes = S.stream_blocks(), S.resident_blocks(), S.stream_passes()
     R = M.Recon(S, prob.y, mode=mode)
     R.run(1)
     M.device_sync()
@@ -180,12 +181,12 @@ def streamed_run(A, prob, block_mib: int, iters: int, plain_value: float,
     xh = M.mlem(A, prob.y, None, iters + 1, mode="fast")
     d = np.abs(xs.astype(np.float64) - xh.astype(np.float64))
     floor = 1e-6 * float(np.max(np.abs(xh)))
-    if mode == "atomic":
+    if mode == "atomic" or passes == 1:
         moved = (8 * m.nnz + 4 * (m.rows + nb)) * (nb - nr) / nb
     else:
         moved = (16 * m.nnz + 4 * (m.rows + nb * (m.cols + 1))) * (nb - nr) / nb
     return {"mode": mode, "block_mib": block_mib, "blocks": nb, "resident_blocks": nr,
-            "iterations": iters,
+            "passes_per_iteration": 1 if mode == "atomic" else passes, "iterations": iters,
             "ms_per_iteration": 1e3 * wall / iters, "iterations_per_s": iters / wall,
             "h2d_bytes_per_iteration": moved, "h2d_gbs": moved * iters / wall / 1e9,
             "slowdown_vs_hbm": plain_value / (iters / wall), "open_s": open_s,

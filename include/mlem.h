@@ -134,7 +134,9 @@ MLEM_API int mlem_open_host(uint32_t rows, uint32_t cols, uint64_t nnz,
  * resident_bytes are also kept in device memory (0: none). Every forward
  * projection streams the CSR parts of the other blocks and every
  * backprojection their CSC parts over PCIe through two device buffers, the
- * copy of one block overlapping the kernels of the block before.
+ * copy of one block overlapping the kernels of the block before. A
+ * reconstruction iteration streams each CSR part once and builds the block's
+ * CSC on the device right after its forward projection (mlem_stream_passes).
  * A column's backprojection continues its running sum from block to block in
  * ascending row order, so EXACT results equal the in-core ones bit for bit.
  * In a reconstruction in MLEM_MODE_ATOMIC each block's CSR part crosses PCIe
@@ -162,6 +164,12 @@ MLEM_API int mlem_open_host_streamed(uint32_t rows, uint32_t cols, uint64_t nnz,
  * device memory (resident may be NULL); 0 for a matrix held in HBM. No
  * reference counterpart (version_7.cu prints its part count). */
 MLEM_API int mlem_stream_blocks(const mlem_matrix* m, uint32_t* blocks, uint32_t* resident);
+/* Passes over the streamed blocks per FAST / EXACT reconstruction iteration:
+ * 1 when each block's CSC is built on the device from its CSR part (the
+ * usual case), 2 when that did not fit in device memory at open and the
+ * host-built CSC parts are streamed too; 0 for a matrix held in HBM. ATOMIC
+ * iterations always make one pass. No reference counterpart. */
+MLEM_API int mlem_stream_passes(const mlem_matrix* m, int* passes);
 
 /* Release device memory, stream and handle. Replaces Csr4Matrix::~Csr4Matrix
  * (csr4matrix.cpp:159-163). Accepts NULL. */
@@ -259,7 +267,10 @@ MLEM_API int mlem_recon_run(mlem_recon* r, int iterations);
  * kernel_bwd: backprojection+update launches (ATOMIC mode: scatter and
  * update summed; a streamed matrix in ATOMIC mode scatters each row block
  * right after its forward projection, so kernel_fwd holds both and
- * kernel_bwd the update). */
+ * kernel_bwd the update; in FAST / EXACT mode a streamed matrix is timed
+ * with its two passes, the forward over the CSR parts and the
+ * backprojection over the CSC parts, same results as the one-pass
+ * mlem_recon_run). */
 MLEM_API int mlem_recon_run_timed(mlem_recon* r, int iterations, float* kernel_fwd_ms,
                          float* kernel_bwd_ms, float* span_ms);
 /* Copy the current image to the host. */

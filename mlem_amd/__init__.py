@@ -152,6 +152,13 @@ class Csr4Matrix:
         check(lib().mlem_stream_blocks(self._h, C.byref(n), C.byref(r)))
         return n.value
 
+    def stream_passes(self) -> int:
+        """Passes over the streamed blocks per FAST / EXACT iteration (1: each
+        block's CSC built on the device; 2: CSC parts streamed too; 0: in HBM)."""
+        n = C.c_int()
+        check(lib().mlem_stream_passes(self._h, C.byref(n)))
+        return n.value
+
     def resident_blocks(self) -> int:
         """Blocks of a streamed matrix kept in device memory."""
         n, r = C.c_uint32(), C.c_uint32()

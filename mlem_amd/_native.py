@@ -79,6 +79,7 @@ SIGNATURES = {
     "mlem_open_host_streamed": (C.c_int, [_u32, _u32, _u64, _P, _P, _P, C.c_uint, _u64, _u64,
                                           C.POINTER(_P)]),
     "mlem_stream_blocks": (C.c_int, [_P, _PU32, _PU32]),
+    "mlem_stream_passes": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "mlem_close": (None, [_P]),
     "mlem_dims": (C.c_int, [_P, _PU32, _PU32, C.POINTER(C.c_uint64)]),
     "mlem_geometry": (C.c_int, [_P, _PU32, _PU32]),

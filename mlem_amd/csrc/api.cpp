@@ -455,6 +455,8 @@ void enqueue_iteration(mlem_recon* R) {
       launch_scatter(m, R->r, R->u, m.stream);
     }
     launch_update(m.cols, R->u, m.norm, R->x, true, m.stream);
+  } else if (m.streamed) {
+    stream_iteration(m, R->mode, R->x, R->y, R->r, st, m.stream);
   } else {
     launch_fwd(m, R->mode, R->x, R->y, R->r, true, st, m.stream);
     launch_bwd(m, R->mode, R->r, m.norm, R->x, true, m.stream);
@@ -860,6 +862,14 @@ int mlem_open_host_streamed(uint32_t rows, uint32_t cols, uint64_t nnz, const ui
           std::memcpy(dst, src + 8 * first, 8 * count);
         },
         grid, flags, block_bytes, resident_bytes);
+  });
+}
+
+int mlem_stream_passes(const mlem_matrix* h, int* passes) {
+  return guarded([&] {
+    if (!h || !passes) raise(MLEM_EINVAL, "null argument");
+    const StreamedMatrix* S = h->m.streamed;
+    *passes = !S ? 0 : S->csc_dev ? 1 : 2;
   });
 }
 

@@ -111,6 +111,13 @@ struct StreamedMatrix {
   hipEvent_t freed[2] = {nullptr, nullptr};   // kernels reading dev[i] done
   float* u = nullptr;                          // [cols] backprojection sums
   int lanes_fwd = 16, lanes_bwd = 4;           // FAST lane groups per row / column
+  // A streamed block's CSC built on the device from its CSR part (colptr,
+  // elements, sort scratch for the largest streamed block), so that a
+  // reconstruction iteration streams each block once; null when it did not
+  // fit in device memory next to the two part buffers (two passes then).
+  char* csc_dev = nullptr;
+  uint64_t csc_scratch_bytes = 0;
+  uint64_t max_stream_nnz = 0;
   StreamedMatrix() = default;
   StreamedMatrix(const StreamedMatrix&) = delete;
   StreamedMatrix& operator=(const StreamedMatrix&) = delete;
@@ -229,6 +236,20 @@ void stream_bwd_add(const DevMatrix& m, bool exact, const float* r, float* u, hi
 // iteration instead of twice).
 void stream_fwd_scatter(const DevMatrix& m, const float* x, const int32_t* y, float* r, float* u,
                         const StatsState* st, hipStream_t s);
+// One FAST / EXACT reconstruction iteration over a streamed matrix: per block
+// the forward projection with the fused ratio into r, then (with csc_dev) the
+// block's CSC built on the device and its backprojection added to u, so each
+// CSR part crosses PCIe once; without csc_dev the two passes of stream_fwd and
+// stream_bwd. Then x *= (n != 0) ? u/n : u.
+void stream_iteration(const DevMatrix& m, int mode, float* x, const int32_t* y, float* r,
+                      const StatsState* st, hipStream_t s);
+// CSC of one row block (block-local row pointers, global columns) built on the
+// device: a stable sort by column keeps the block-local rows ascending inside
+// every column. block_csc_scratch gives the scratch bytes it needs.
+size_t block_csc_scratch(uint64_t nnz, uint32_t cols);
+void build_block_csc(const uint32_t* rowptr, const uint2* csr, uint32_t rows, uint32_t cols,
+                     uint64_t nnz, void* scratch, size_t scratch_bytes, uint32_t* colptr,
+                     uint2* csc, hipStream_t s);
 // x *= (n != 0) ? u/n : u; optionally u = 0 afterwards.
 void launch_update(uint32_t cols, const float* u, const float* norm, float* x,
                    bool zero_u, hipStream_t s);

@@ -269,6 +269,49 @@ std::vector<uint32_t> brick_colmap(uint32_t nx, uint32_t ny, uint32_t nz) {
   return key;
 }
 
+namespace {
+size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
+size_t block_csc_sort_bytes(uint64_t nnz, uint32_t cols) {
+  size_t tb = 0;
+  MLEM_HIP(rocprim::radix_sort_pairs(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                     (unsigned long long*)nullptr, (unsigned long long*)nullptr,
+                                     (size_t)nnz, 0, bits_for(cols ? cols - 1 : 0),
+                                     hipStream_t(nullptr)));
+  return tb;
+}
+}  // namespace
+
+size_t block_csc_scratch(uint64_t nnz, uint32_t cols) {
+  return up256(4 * nnz) * 2 + up256(8 * nnz) + up256(block_csc_sort_bytes(nnz, cols));
+}
+
+void build_block_csc(const uint32_t* rowptr, const uint2* csr, uint32_t rows, uint32_t cols,
+                     uint64_t nnz, void* scratch, size_t scratch_bytes, uint32_t* colptr,
+                     uint2* csc, hipStream_t s) {
+  if (nnz == 0) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for((uint64_t)cols + 1)), dim3(kBlock), 0, s, colptr,
+                       0u, (uint64_t)cols + 1);
+    check_launch("block colptr fill");
+    return;
+  }
+  char* p = static_cast<char*>(scratch);
+  uint32_t* keys = reinterpret_cast<uint32_t*>(p);
+  uint32_t* skeys = reinterpret_cast<uint32_t*>(p + up256(4 * nnz));
+  auto* vals = reinterpret_cast<unsigned long long*>(p + 2 * up256(4 * nnz));
+  void* temp = p + 2 * up256(4 * nnz) + up256(8 * nnz);
+  size_t tb = block_csc_sort_bytes(nnz, cols);
+  if (2 * up256(4 * nnz) + up256(8 * nnz) + tb > scratch_bytes)
+    raise(MLEM_ESTATE, "block CSC scratch too small");
+  hipLaunchKernelGGL(k_csc_keys, dim3(blocks_for((uint64_t)rows * 16)), dim3(kBlock), 0, s, rowptr,
+                     csr, rows, (const uint32_t*)nullptr, (const uint32_t*)nullptr, keys, vals);
+  check_launch("block csc key launch");
+  MLEM_HIP(rocprim::radix_sort_pairs(temp, tb, keys, skeys, vals,
+                                     reinterpret_cast<unsigned long long*>(csc), (size_t)nnz, 0,
+                                     bits_for(cols ? cols - 1 : 0), s));
+  hipLaunchKernelGGL(k_colptr, dim3(blocks_for(nnz)), dim3(kBlock), 0, s, skeys, nnz, cols, colptr);
+  check_launch("block colptr launch");
+}
+
 void build_layout(DevMatrix& m, bool reorder_rows, bool reorder_cols, bool want_csc,
                   hipStream_t s) {
   const bool geom = m.grid[0] && m.grid[1] && m.grid[2] &&

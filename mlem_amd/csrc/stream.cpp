@@ -40,6 +40,7 @@ StreamedMatrix::~StreamedMatrix() {
     if (dev[i]) (void)hipFree(dev[i]);
   }
   if (u) (void)hipFree(u);
+  if (csc_dev) (void)hipFree(csc_dev);
   if (copy) (void)hipStreamDestroy(copy);
   if (host) (void)hipHostFree(host);
 }
@@ -161,6 +162,7 @@ StreamedMatrix* build_streamed(uint32_t rows, uint32_t cols, uint64_t nnz,
       kept += both;
     } else {
       streamed_part = std::max(streamed_part, std::max(B.csr_bytes, B.csc_bytes));
+      S->max_stream_nnz = std::max(S->max_stream_nnz, B.nnz);
     }
   }
   S->part_bytes = streamed_part;
@@ -173,6 +175,16 @@ StreamedMatrix* build_streamed(uint32_t rows, uint32_t cols, uint64_t nnz,
   MLEM_HIP(hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking));
   MLEM_HIP(hipMalloc((void**)&S->u, 4ull * std::max<uint32_t>(cols, 1)));
   MLEM_HIP(hipMemsetAsync(S->u, 0, 4ull * std::max<uint32_t>(cols, 1), s));
+  if (S->max_stream_nnz > 0) {
+    // device CSC of one streamed block; if it does not fit, iterations keep
+    // streaming the host-built CSC parts
+    S->csc_scratch_bytes = block_csc_scratch(S->max_stream_nnz, cols);
+    const uint64_t bytes = colpart + align16(8 * S->max_stream_nnz) + S->csc_scratch_bytes;
+    if (hipMalloc((void**)&S->csc_dev, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      S->csc_dev = nullptr;
+    }
+  }
   // lane groups of about four elements per lane (4..64), as the in-core
   // lane-group kernels choose them
   auto lanes_for = [](double avg) {
@@ -261,6 +273,42 @@ void stream_fwd_scatter(const DevMatrix& m, const float* x, const int32_t* y, fl
         if (B.rows) launch_scatter_block(rp, el, B.rows, S.lanes_fwd, r + B.row0, u, m.cus, s);
       },
       s);
+}
+
+void stream_iteration(const DevMatrix& m, int mode, float* x, const int32_t* y, float* r,
+                      const StatsState* st, hipStream_t s) {
+  const StreamedMatrix& S = *m.streamed;
+  if (!S.csc_dev) {
+    stream_fwd(m, mode, x, y, r, true, st, s);
+    stream_bwd(m, mode, r, m.norm, x, true, s);
+    return;
+  }
+  const bool exact = mode == MLEM_MODE_EXACT;
+  const uint64_t colpart = align16(4ull * ((uint64_t)m.cols + 1));
+  uint32_t* colptr = reinterpret_cast<uint32_t*>(S.csc_dev);
+  uint2* csc = reinterpret_cast<uint2*>(S.csc_dev + colpart);
+  void* scratch = S.csc_dev + colpart + align16(8 * S.max_stream_nnz);
+  launch_fill(S.u, 0.f, m.cols, s);
+  stream_pass(
+      m, false,
+      [&](const StreamBlock& B, char* d) {
+        const uint32_t* rp = reinterpret_cast<const uint32_t*>(d);
+        const uint2* el = reinterpret_cast<const uint2*>(d + align16(4ull * ((uint64_t)B.rows + 1)));
+        launch_fwd_block(rp, el, B.rows, S.lanes_fwd, exact, x, y + B.row0, r + B.row0, true, st,
+                         m.cus, s);
+        if (B.resident) {  // its CSC part is in device memory already (global rows)
+          const char* c = B.resident + align16(B.csr_bytes);
+          launch_bwd_block_add(reinterpret_cast<const uint32_t*>(c),
+                               reinterpret_cast<const uint2*>(c + colpart), m.cols, S.lanes_bwd,
+                               exact, r, S.u, m.cus, s);
+        } else {  // block-local rows
+          build_block_csc(rp, el, B.rows, m.cols, B.nnz, scratch, S.csc_scratch_bytes, colptr, csc,
+                          s);
+          launch_bwd_block_add(colptr, csc, m.cols, S.lanes_bwd, exact, r + B.row0, S.u, m.cus, s);
+        }
+      },
+      s);
+  launch_update(m.cols, S.u, m.norm, x, false, s);
 }
 
 void stream_bwd(const DevMatrix& m, int mode, const float* r, const float* norm, float* out,

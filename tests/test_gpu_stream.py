@@ -40,6 +40,8 @@ def small_streamed():
 def test_blocks_and_fixed_launch(small_streamed):
     p, A = small_streamed
     assert A.stream_blocks() >= 5
+    # iterations make one pass: each block's CSC is built on the device
+    assert A.stream_passes() == 1
     with pytest.raises(MlemError):
         A.set_launch(("dict512", 32), ("dict512", 32))
     with pytest.raises(MlemError):
@@ -108,7 +110,9 @@ def test_recon_incremental_matches_one_shot(small_streamed):
 
 
 def test_timed_run(small_streamed):
-    # per-kernel HIP-event timing covers every block's copy and kernels
+    # per-kernel HIP-event timing covers every block's copy and kernels; the
+    # timed run makes the two passes (each projection timed), the plain run
+    # one pass with each block's CSC built on the device: the same bits
     p, A = small_streamed
     R = M.Recon(A, p.y, mode="exact")
     kf, kb, span = R.run_timed(3)
