@@ -129,7 +129,7 @@ MLEM_API int mlem_open_host(uint32_t rows, uint32_t cols, uint64_t nnz,
 
 /* Out-of-core matrices (larger than HBM). The elements stay in pinned host
  * memory in blocks of whole rows; a block holds its rows (CSR) and their
- * transpose (CSC), each part at most block_bytes bytes (0: a quarter of the
+ * transpose (CSC), each part at most block_bytes bytes (0: an eighth of the
  * device memory free at open). The first blocks whose parts fit in
  * resident_bytes are also kept in device memory (0: none). Every forward
  * projection streams the CSR parts of the other blocks and every

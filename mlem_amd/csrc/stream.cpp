@@ -50,11 +50,13 @@ StreamedMatrix* build_streamed(uint32_t rows, uint32_t cols, uint64_t nnz,
                                uint64_t block_bytes, uint64_t resident_bytes, hipStream_t s) {
   const uint64_t colpart = align16(4ull * ((uint64_t)cols + 1));
   if (block_bytes == 0) {
-    // a quarter of the free device memory per buffer, as the reference sizes
-    // its parts by cudaMemGetInfo (helper_v_7_8.cu:6-30)
+    // an eighth of the free device memory per part, as the reference sizes
+    // its parts by cudaMemGetInfo (helper_v_7_8.cu:6-30): the two part
+    // buffers and one block's device CSC with its sort scratch (about four
+    // parts) then fit
     size_t freeb = 0, total = 0;
     MLEM_HIP(hipMemGetInfo(&freeb, &total));
-    block_bytes = freeb / 4;
+    block_bytes = freeb / 8;
   }
   auto row_len = [&](uint32_t r) { return rowidx[r] - (r ? rowidx[r - 1] : 0); };
 
