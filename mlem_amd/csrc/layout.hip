@@ -192,6 +192,24 @@ __global__ __launch_bounds__(kBlock) void k_colptr(const uint32_t* __restrict__ 
     for (uint64_t cc = (uint64_t)c + 1; cc <= cols; ++cc) colptr[cc] = (uint32_t)nnz;
 }
 
+// colptr[c] = first position with key >= c, by binary search (one lane per
+// column, c = 0..cols): no lane walks a long run of empty columns.
+__global__ __launch_bounds__(kBlock) void k_colptr_search(const uint32_t* __restrict__ skeys,
+                                                          uint64_t nnz, uint32_t cols,
+                                                          uint32_t* __restrict__ colptr) {
+  const uint64_t c = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
+  if (c > cols) return;
+  uint64_t lo = 0, hi = nnz;  // answer in [lo, hi]
+  while (lo < hi) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (skeys[mid] < c)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  colptr[c] = (uint32_t)lo;
+}
+
 __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * kBlock)
@@ -308,7 +326,8 @@ void build_block_csc(const uint32_t* rowptr, const uint2* csr, uint32_t rows, ui
   MLEM_HIP(rocprim::radix_sort_pairs(temp, tb, keys, skeys, vals,
                                      reinterpret_cast<unsigned long long*>(csc), (size_t)nnz, 0,
                                      bits_for(cols ? cols - 1 : 0), s));
-  hipLaunchKernelGGL(k_colptr, dim3(blocks_for(nnz)), dim3(kBlock), 0, s, skeys, nnz, cols, colptr);
+  hipLaunchKernelGGL(k_colptr_search, dim3(blocks_for((uint64_t)cols + 1)), dim3(kBlock), 0, s,
+                     skeys, nnz, cols, colptr);
   check_launch("block colptr launch");
 }
 
