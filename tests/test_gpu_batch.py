@@ -104,14 +104,16 @@ def test_batch_incremental_runs(tiny_batch):
     B.close()
 
 
-def test_batch_small_workload():
-    # 32^3 grid, 262,144 lines: 6 frames, budgets up to 12, width 4 (two blocks)
+@pytest.mark.parametrize("width", [4, 0])
+def test_batch_small_workload(width):
+    # 32^3 grid, 262,144 lines: 6 frames, budgets up to 12; width 4 (two
+    # blocks) or autotuned (blocks and lone frames by measured cost)
     p = synth.make_problem("small")
     m = p.matrix
     A = M.Csr4Matrix.from_arrays(m.rows, m.cols, m.rowidx, m.elements, grid=m.grid)
     Y = make_frames(p, 6, 1e5, 1e7)
     budgets = np.array([12, 3, 7, 1, 10, 5], np.int32)
-    B = M.Batch(A, Y, budgets, block_width=4)
+    B = M.Batch(A, Y, budgets, block_width=width)
     B.run(50)
     for f in range(6):
         xo = orc.mlem(m.rowidx, m.elements, m.cols, Y[f], int(budgets[f]))[0]
