@@ -42,6 +42,76 @@ def kernel_bytes(rows: int, cols: int, nnz: int, precision: str = "fp32") -> dic
     return {"fwd": fwd, "bwd": bwd, "iteration": fwd + bwd}
 
 
+def quarter_means(t) -> list:
+    """Mean kernel time of the first and of the last quarter of the timed
+    launches (clock ramp versus steady state)."""
+    t = np.asarray(t, np.float64)
+    q = max(1, t.size // 4)
+    return [float(np.mean(t[:q])), float(np.mean(t[-q:]))]
+
+
+def large_run(steps: int, fp64_iters: int) -> dict:
+    """The 192x192x64 / 4,194,304-line matrix (about 0.79 G nonzeros) with fp32,
+    fp16 and bf16 values (16-bit: per-row fp32 scale): iterations/s, per-kernel
+    times and roofline fractions after `steps` timed FAST iterations, and the
+    image after `fp64_iters` iterations against FP64 MLEM on the original
+    values (relative L2 and L-infinity)."""
+    import mlem_amd as M
+    from mlem_amd import synth
+    from oracle import oracle as orc  # accuracy reference only
+    t0 = time.time()
+    prob = synth.make_problem("large")
+    m = prob.matrix
+    out = {"workload": f"large: {'x'.join(map(str, m.grid))} grid, {m.rows} lines, "
+                       f"{m.nnz} nonzeros", "rows": m.rows, "cols": m.cols, "nnz": m.nnz,
+           "generation_s": time.time() - t0, "steps": steps}
+    t0 = time.time()
+    x64 = orc.f64_mlem_par(m.rowidx, m.elements, m.cols, prob.y.astype(np.float64),
+                           fp64_iters)[0] if fp64_iters > 0 else None
+    out["fp64_iterations"] = fp64_iters
+    out["fp64_oracle_s"] = time.time() - t0
+    for prec in ("fp32", "fp16", "bf16"):
+        t0 = time.time()
+        A = M.Csr4Matrix.from_arrays(m.rows, m.cols, m.rowidx, m.elements, grid=m.grid,
+                                     precision=prec)
+        open_s = time.time() - t0
+        try:
+            t0 = time.time()
+            A.autotune()
+            tune_s = time.time() - t0
+            R = M.Recon(A, prob.y, mode="fast", use_graph=False)
+            R.run(3)
+            M.device_sync()
+            t0 = time.perf_counter()
+            kf, kb, _ = R.run_timed(steps)
+            M.device_sync()
+            wall = time.perf_counter() - t0
+            R.close()
+            B = kernel_bytes(m.rows, m.cols, m.nnz, prec)
+            f_ms, b_ms = float(np.mean(kf)), float(np.mean(kb))
+            res = {"iterations_per_s": steps / wall, "ms_per_iteration": 1e3 * wall / steps,
+                   "launch": A.launch(), "open_s": open_s, "autotune_s": tune_s,
+                   "kernel_ms": {"fwd": f_ms, "bwd": b_ms},
+                   "roofline_frac": {"fwd": B["fwd"] / (f_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                                     "bwd": B["bwd"] / (b_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                                     "iteration": B["iteration"] / (wall / steps) / 1e9
+                                                  / PEAK_HBM_GBS},
+                   "algorithmic_bytes": B}
+            if x64 is not None:
+                x = M.mlem(A, prob.y, None, fp64_iters, mode="fast").astype(np.float64)
+                floor = 1e-6 * float(np.max(np.abs(x64)))
+                res["vs_fp64_original"] = {
+                    "rel_l2": float(np.linalg.norm(x - x64) / np.linalg.norm(x64)),
+                    "rel_linf": float(np.max(np.abs(x - x64) / np.maximum(np.abs(x64), floor)))}
+        finally:
+            A.close()
+        out[prec] = res
+        print(f"[bench] large {prec}: {res}", file=sys.stderr, flush=True)
+    f32 = out["fp32"]["iterations_per_s"]
+    out["speedup_vs_fp32"] = {p: out[p]["iterations_per_s"] / f32 for p in ("fp16", "bf16")}
+    return out
+
+
 def unrolled_run(A, prob, frames: int, K: int, checkpoint: int) -> dict:
     """Differentiable unrolled MLEM on the same matrix: `frames` sinograms
     with relaxed (real) counts, K iterations from x0 = sum(y)/sum(n), loss
@@ -193,25 +263,24 @@ def streamed_run(A, prob, block_mib: int, iters: int, plain_value: float,
             "rel_linf_vs_in_core": float(np.max(d / np.maximum(np.abs(xh), floor)))}
 
 
-def batched_run(A, prob, frames: int, width: int, plain_value: float) -> dict:
+def batched_run(A, prob, frames: int, width: int, plain_value: float, mode: str = "fast",
+                parity: bool = True, Y=None) -> dict:
     """Dynamic-frame workload on the same matrix: per-frame phantom scaling
     U(0.2, 2) per ellipsoid, total counts log-uniform in [1e5, 1e8], iteration
     budgets uniform in 10..100, early stop at a relative log-likelihood change
-    below 1e-6. Returns frame-iterations/s and the parity of the frames with
-    the fewest and the most iterations against the fp32 sequential
-    restatement."""
+    below 1e-6. Returns frame-iterations/s and, with `parity`, the frames with
+    the fewest and the most iterations against FP64 MLEM (and, in EXACT mode,
+    their bit-identity with the fp32 sequential restatement)."""
     import mlem_amd as M
     from mlem_amd import synth
-    w = synth.WORKLOADS[prob.name]
     t0 = time.time()
-    counts = synth.frame_counts(frames)
     budgets = synth.frame_budgets(frames)
-    Y = np.empty((frames, prob.matrix.rows), np.int32)
-    for f, sc in enumerate(synth.frame_scales(frames, w["ellipsoids"])):
-        x = synth.make_phantom(w["grid"], w["ellipsoids"], 2000 + w["key"], w["semi"], scale=sc)
-        Y[f] = synth.poisson(synth.project(prob.matrix, x), counts[f], 5000 + f)
+    if Y is None:
+        Y = synth.dynamic_frames(prob, frames)
     gen = time.time() - t0
-    B = M.Batch(A, Y, budgets, rel_tol=1e-6, block_width=width)
+    t0 = time.time()
+    B = M.Batch(A, Y, budgets, rel_tol=1e-6, block_width=width, mode=mode)
+    create_s = time.time() - t0
     B.run(1)  # warm-up: graph capture, first launches
     done0 = B.progress()["frame_iterations"]
     M.device_sync()
@@ -221,7 +290,7 @@ def batched_run(A, prob, frames: int, width: int, plain_value: float) -> dict:
     wall = time.perf_counter() - t0
     fi = B.progress()["frame_iterations"] - done0
     its = [B.frame_info(f)[0] for f in range(frames)]
-    out = {"frames": frames, "block_width": width or "autotuned",
+    out = {"frames": frames, "mode": mode, "block_width": width or "autotuned",
            "block_ms_by_width": B.tuning() if width == 0 else None,
            "frame_iterations": int(fi),
            "batched_iterations": n + 1, "wall_s": wall,
@@ -230,16 +299,30 @@ def batched_run(A, prob, frames: int, width: int, plain_value: float) -> dict:
            "iterations_per_frame": {"min": int(min(its)), "max": int(max(its)),
                                     "mean": float(np.mean(its))},
            "early_stopped_frames": int(sum(1 for f in range(frames) if its[f] < budgets[f])),
-           "generation_s": gen}
-    from oracle import oracle as orc  # parity check only
-    m = prob.matrix
-    out["parity"] = []
-    # the frame that stopped first and the one that ran longest (last, alone)
-    for f in (int(np.argmin(its)), int(np.argmax(its))):
-        xo = orc.mlem(m.rowidx, m.elements, m.cols, Y[f], its[f])[0]
-        out["parity"].append({"frame": f, "iterations": its[f],
-                              "bit_identical_to_seq_fp32": bool(np.array_equal(B.image(f), xo))})
+           "generation_s": gen, "create_s": create_s,
+           "note": "the timed span starts after the first batched iteration (graph "
+                   "capture); create_s includes building the matrix's tiling and timing "
+                   "the block widths"}
+    if parity:
+        from oracle import oracle as orc  # parity check only
+        m = prob.matrix
+        out["parity"] = []
+        # the frame that stopped first and the one that ran longest (last)
+        for f in (int(np.argmin(its)), int(np.argmax(its))):
+            x = B.image(f)
+            x64 = orc.f64_mlem_par(m.rowidx, m.elements, m.cols, Y[f].astype(np.float64),
+                                   its[f])[0]
+            floor = 1e-6 * float(np.max(np.abs(x64)))
+            d = np.abs(x.astype(np.float64) - x64)
+            rec = {"frame": f, "iterations": its[f],
+                   "rel_linf_vs_fp64": float(np.max(d / np.maximum(np.abs(x64), floor))),
+                   "rel_l2_vs_fp64": float(np.linalg.norm(d) / np.linalg.norm(x64))}
+            if mode == "exact":
+                xo = orc.mlem(m.rowidx, m.elements, m.cols, Y[f], its[f])[0]
+                rec["bit_identical_to_seq_fp32"] = bool(np.array_equal(x, xo))
+            out["parity"].append(rec)
     B.close()
+    out["_Y"] = Y
     return out
 
 
@@ -255,7 +338,9 @@ def main() -> int:
     ap.add_argument("--parity-iters", type=int, default=60,
                     help="iterations of the FAST / EXACT image checked against the fp32 "
                          "sequential restatement (0 = skip)")
-    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
+                    help="pick each projection's kernel family and tile size by timing them "
+                         "on the matrix (mlem_autotune) before the warm-up")
     ap.add_argument("--keep-order", action="store_true",
                     help="keep file row/voxel order on the device (no locality layout)")
     ap.add_argument("--launch", default="", help="family,param,family,param e.g. tiles,8,groups,32")
@@ -263,6 +348,11 @@ def main() -> int:
                     help="frames of the batched (dynamic-frame) measurement; 0 = skip")
     ap.add_argument("--batched-width", type=int, default=0,
                     help="widest RHS block (4..32); 0 = autotuned widths")
+    ap.add_argument("--batched-mode", default="fast", choices=["fast", "exact"],
+                    help="fast: the matrix's two-dimensional tiling (within 1e-4 of FP64); "
+                         "exact: every frame bit-identical to the sequential driver")
+    ap.add_argument("--batched-exact-too", action=argparse.BooleanOptionalAction, default=True,
+                    help="also time the bit-identical (EXACT) batched mode")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="first run the plain copy kernel over this many MiB (calibrates "
                          "hardware byte counters under rocprofv3 --pmc)")
@@ -275,6 +365,11 @@ def main() -> int:
                     help="iterations of the out-of-core (host-streamed) measurement; 0 = skip")
     ap.add_argument("--streamed-block-mib", type=int, default=256,
                     help="largest part of a streamed row block (MiB)")
+    ap.add_argument("--large-steps", type=int, default=20,
+                    help="timed iterations per value precision on the 192x192x64 matrix "
+                         "(fp32, fp16, bf16); 0 = skip")
+    ap.add_argument("--large-fp64-iters", type=int, default=5,
+                    help="iterations of the 192x192x64 images checked against FP64 MLEM")
     ap.add_argument("--fp64-iters", type=int, default=0,
                     help="also compare this many iterations with FP64 MLEM (oracle, all "
                          "host cores); 0 = skip")
@@ -308,8 +403,13 @@ def main() -> int:
     if args.launch:
         f, fp, b, bp = args.launch.split(",")
         A.set_launch((f, int(fp)), (b, int(bp)))
-    if args.autotune:
+    tune_s = None
+    if args.autotune and not args.launch:
+        t0 = time.time()
         A.autotune()
+        tune_s = time.time() - t0
+        print(f"[bench] autotuned launch {A.launch()} in {tune_s:.1f}s", file=sys.stderr,
+              flush=True)
     launch = A.launch()
 
     R = M.Recon(A, prob.y, mode=args.mode, use_graph=False)
@@ -337,9 +437,9 @@ def main() -> int:
     if args.mode == "atomic" and dom == "bwd":
         name = "k_scatter + k_update (atomic backprojection)"
 
-    # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
-    # FETCH_SIZE/WRITE_SIZE passes (copy-kernel calibrated) of this workload
-    # and launch configuration.
+    # HBM bytes per launch of the dominant kernel: the L2-to-memory read and
+    # write requests of the committed rocprofv3 counter passes for this
+    # workload and launch configuration, times their request sizes.
     traffic = None
     ptag = "" if args.precision == "fp32" else f"_{args.precision}"
     tfile = ROOT / "profiles" / f"traffic_{args.workload}_{args.mode}{ptag}.json"
@@ -360,7 +460,15 @@ def main() -> int:
     batched = None
     if args.batched_frames > 0 and args.mode != "atomic":
         try:
-            batched = batched_run(A, prob, args.batched_frames, args.batched_width, value)
+            batched = batched_run(A, prob, args.batched_frames, args.batched_width, value,
+                                  args.batched_mode)
+            Y = batched.pop("_Y")
+            if args.batched_exact_too and args.batched_mode != "exact":
+                ex = batched_run(A, prob, args.batched_frames, args.batched_width, value,
+                                 "exact", parity=False, Y=Y)
+                batched["exact_mode"] = {k: ex[k] for k in (
+                    "frame_iterations_per_s", "speedup_vs_plain", "frame_iterations", "wall_s",
+                    "block_ms_by_width")}
         except Exception as e:  # reported in the line, never hides the plain result
             batched = {"error": f"{type(e).__name__}: {e}"}
         print(f"[bench] batched: {batched}", file=sys.stderr, flush=True)
@@ -384,6 +492,14 @@ def main() -> int:
         except Exception as e:  # reported in the line, never hides the plain result
             streamed = {"error": f"{type(e).__name__}: {e}"}
         print(f"[bench] streamed: {streamed}", file=sys.stderr, flush=True)
+
+    large = None
+    if args.large_steps > 0 and args.workload == "pet" and args.mode == "fast" \
+            and args.precision == "fp32":
+        try:
+            large = large_run(args.large_steps, args.large_fp64_iters)
+        except Exception as e:  # reported in the line, never hides the plain result
+            large = {"error": f"{type(e).__name__}: {e}"}
 
     accuracy = None
     if args.fp64_iters > 0:
@@ -411,7 +527,14 @@ def main() -> int:
             "precision": args.precision, "launch": launch, "iterations_run": done,
         },
         "hbm_gbs_iteration": B["iteration"] / (ms_per_step * 1e-3) / 1e9,
-        "kernel_ms": {"fwd": fwd_ms, "bwd": bwd_ms, "span_per_iter": span / args.steps},
+        "kernel_ms": {"fwd": fwd_ms, "bwd": bwd_ms, "span_per_iter": span / args.steps,
+                      "fwd_min_median_max": [float(np.min(kf)), float(np.median(kf)),
+                                             float(np.max(kf))],
+                      "bwd_min_median_max": [float(np.min(kb)), float(np.median(kb)),
+                                             float(np.max(kb))],
+                      "fwd_first_last_quarter": quarter_means(kf),
+                      "bwd_first_last_quarter": quarter_means(kb)},
+        "autotune_s": tune_s,
         "roofline": {
             "kernel": name,
             "bound": "hbm",
@@ -429,6 +552,7 @@ def main() -> int:
         "batched": batched,
         "unrolled": unrolled,
         "streamed": streamed,
+        "large": large,
         "fp64_check": accuracy,
     }
     print(json.dumps(line), flush=True)

@@ -221,11 +221,19 @@ MLEM_API int mlem_update(mlem_matrix* m, const float* u, const float* norm, floa
 
 #define MLEM_INIT_REFERENCE 0 /* x0 = (float)(sum y / sum norm), initImage */
 #define MLEM_INIT_GIVEN 1     /* x0 = caller's x_inout                     */
+#define MLEM_CHECKPOINT_RESTORE_ONLY (-1) /* see mlem_options.checkpoint_interval */
 
 typedef struct mlem_options {
   int mode;               /* MLEM_MODE_*                                  */
   int init;               /* MLEM_INIT_*                                  */
-  int checkpoint_interval;/* >0: write img.chkpt/iter.chkpt every N iters */
+  int checkpoint_interval;/* 0: no checkpoint files (seqcsr4mlem);
+                             >0: restart from img.chkpt/iter.chkpt when both
+                             exist, write them every N iterations, remove
+                             them at the end;
+                             MLEM_CHECKPOINT_RESTORE_ONLY: restart from them
+                             when both exist, write and remove none (the
+                             reference's <checkpointing> = 0,
+                             mpicsr4mlem.cpp:536-549,589,657)            */
   const char* checkpoint_dir; /* directory of the checkpoint files (NULL = CWD) */
   int verbose;            /* 1: print the reference driver's stdout lines */
   int use_graph;          /* 1: run iterations from a captured hipGraph   */
@@ -281,9 +289,9 @@ MLEM_API int mlem_recon_iterations(const mlem_recon* r);
 /* ---- batched reconstruction: many sinograms, one matrix ------------------
  * Frames (e.g. dynamic PET frames) are reconstructed together in blocks of up
  * to block_width right-hand sides; each matrix record streamed from HBM
- * serves a whole block. Per frame the arithmetic is that of mlem()
- * (seqcsr4mlem.cpp:137-189): the image after k iterations is bit-identical to
- * k iterations of the sequential driver. A frame stops after budgets[f]
+ * serves a whole block. Per frame the iteration is that of mlem()
+ * (seqcsr4mlem.cpp:137-189); in MLEM_MODE_EXACT the image after k iterations
+ * is bit-identical to k iterations of the sequential driver. A frame stops after budgets[f]
  * iterations or, with rel_tol > 0, after iteration k >= 2 once
  * |L(x_{k-1}) - L(x_{k-2})| <= rel_tol * |L(x_{k-1})|, L being the FP64
  * Poisson log-likelihood sum_{f_i != 0} (y_i log f_i - f_i) that iteration k
@@ -295,12 +303,21 @@ typedef struct mlem_batch_options {
   double rel_tol;   /* early-stop tolerance on the log-likelihood (0 = off) */
   int block_width;  /* widest RHS block: 4, 8, 16 or 32 frames;
                        1 = every frame alone on the single-image
-                       MLEM_MODE_EXACT kernels;
+                       kernels of the batch's mode;
                        0 = autotune: one iteration is timed at every width
                        (and alone) at creation and live frames are packed
                        into the blocks that minimise the summed measured
                        time, single frames included                       */
   int use_graph;    /* 1: replay each iteration from a captured hipGraph  */
+  int mode;         /* MLEM_MODE_FAST (default): blocks run through the
+                       matrix's two-dimensional tiling (voxel panels; the
+                       gathered slice of every block stays in one XCD's L2),
+                       fp32 fused multiply-adds summed in a fixed order:
+                       deterministic, within 1e-4 of FP64 MLEM; lone frames
+                       on the single-image FAST kernels.
+                       MLEM_MODE_EXACT: every frame's image is bit-identical
+                       to the sequential driver's (per-frame sequential sums
+                       over the CSR / CSC, lone frames on the EXACT kernels) */
 } mlem_batch_options;
 MLEM_API void mlem_batch_options_default(mlem_batch_options* o);
 /* y: frames*rows int32 counts, frame-major; budgets: frames iteration limits

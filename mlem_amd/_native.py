@@ -60,6 +60,7 @@ class BatchOptions(C.Structure):
         ("rel_tol", C.c_double),
         ("block_width", C.c_int),
         ("use_graph", C.c_int),
+        ("mode", C.c_int),
     ]
 
 

@@ -152,6 +152,7 @@ mlem_matrix::~mlem_matrix() {
     free_dict(m.dbwd);
     delete m.streamed;
     m.streamed = nullptr;
+    free_panels(m);
     if (m.stream) (void)hipStreamDestroy(m.stream);
   }
 }
@@ -1112,10 +1113,14 @@ int mlem_reconstruct(mlem_matrix* h, const int32_t* y, uint32_t rows, float* x, 
     if (verbose)
       std::cout << "Calculated norm, elapsed time: " << h->norm_seconds << "s\n";
 
-    // Restart from a complete checkpoint (mpicsr4mlem.cpp:536-553).
+    if (o.checkpoint_interval < MLEM_CHECKPOINT_RESTORE_ONLY)
+      raise(MLEM_EINVAL, "invalid checkpoint interval " + std::to_string(o.checkpoint_interval));
+    // Restart from a complete checkpoint whenever both files exist, whatever
+    // the interval (mpicsr4mlem.cpp:536-553); an interval of 0 is the
+    // sequential driver, which has no checkpoints.
     int start = 0;
     std::vector<float> restored;
-    if (o.checkpoint_interval > 0) {
+    if (o.checkpoint_interval != 0) {
       const bool hi = file_exists(imgChk), ht = file_exists(iterChk);
       if (hi && ht) {
         restored = read_raw_f32(imgChk);

@@ -185,6 +185,75 @@ __global__ __launch_bounds__(kBlock) void k_bwd_batch(
   spmm::walk<W>(colptr, csc, tiles, ntiles, R, epi);
 }
 
+// Epilogue of the tiled forward product (FAST mode): lane f of row group g
+// adds the row's segment slots in panel order, then ratio r = (f != 0) ? y/f
+// : 0 and the frame's FP64 statistics over rows g, g + groups, ...; one
+// partial per workgroup and frame.
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_tiled_ratio(const uint32_t* __restrict__ rowslot,
+                                                        const float* __restrict__ part,
+                                                        const int32_t* __restrict__ Y,
+                                                        float* __restrict__ R, uint32_t rows,
+                                                        double2* __restrict__ partials) {
+  constexpr uint32_t G = kBlock / W;
+  const uint32_t f = threadIdx.x % W, g = threadIdx.x / W;
+  const uint32_t stride = gridDim.x * G;
+  double ll = 0.0, tot = 0.0;
+  for (uint32_t row = blockIdx.x * G + g; row < rows; row += stride) {
+    uint32_t s = rowslot[row];
+    const uint32_t e = rowslot[row + 1];
+    float F = 0.f;
+    for (; s + 4 <= e; s += 4) {
+      const float a0 = part[(size_t)s * W + f], a1 = part[(size_t)(s + 1) * W + f];
+      const float a2 = part[(size_t)(s + 2) * W + f], a3 = part[(size_t)(s + 3) * W + f];
+      F += a0;
+      F += a1;
+      F += a2;
+      F += a3;
+    }
+    for (; s < e; ++s) F += part[(size_t)s * W + f];
+    const size_t i = (size_t)row * W + f;
+    const int32_t yi = Y[i];
+    R[i] = (F != 0.f) ? (float)yi / F : 0.f;
+    tot += (double)F;
+    if (F > 0.f) ll += (yi != 0 ? (double)yi * log((double)F) : 0.0) - (double)F;
+  }
+  batch_stats_partial<W>(ll, tot, partials);
+}
+
+// Epilogue of the tiled backprojection (FAST mode): u = the voxel's group
+// slots added in chunk order, then x *= (n != 0) ? u/n : u for live lanes.
+// Voxels in panel order.
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_tiled_update(const uint32_t* __restrict__ colslot,
+                                                         const float* __restrict__ part,
+                                                         const float* __restrict__ norm,
+                                                         float* __restrict__ X,
+                                                         const uint32_t* __restrict__ mask,
+                                                         uint32_t cols) {
+  constexpr uint32_t G = kBlock / W;
+  const uint32_t f = threadIdx.x % W, g = threadIdx.x / W;
+  if (!((*mask >> f) & 1u)) return;
+  const uint32_t stride = gridDim.x * G;
+  for (uint32_t c = blockIdx.x * G + g; c < cols; c += stride) {
+    uint32_t s = colslot[c];
+    const uint32_t e = colslot[c + 1];
+    float u = 0.f;
+    for (; s + 4 <= e; s += 4) {
+      const float a0 = part[(size_t)s * W + f], a1 = part[(size_t)(s + 1) * W + f];
+      const float a2 = part[(size_t)(s + 2) * W + f], a3 = part[(size_t)(s + 3) * W + f];
+      u += a0;
+      u += a1;
+      u += a2;
+      u += a3;
+    }
+    for (; s < e; ++s) u += part[(size_t)s * W + f];
+    const float n = norm[c];
+    const size_t i = (size_t)c * W + f;
+    X[i] = X[i] * ((n != 0.f) ? u / n : u);
+  }
+}
+
 // out[c*W + lane] = in[src(c)] for one frame: file-order image <-> block lane.
 __global__ __launch_bounds__(kBlock) void k_lane_in(float* __restrict__ X, int W, int lane,
                                                     const float* __restrict__ file_img,
@@ -226,6 +295,16 @@ __global__ __launch_bounds__(kBlock) void k_lane_move(const float* __restrict__ 
                                                       int ld, uint32_t n) {
   const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
   if (c < n) dst[(size_t)c * Wd + ld] = src[(size_t)c * Ws + ls];
+}
+
+// A frame moved between blocks whose columns are in different orders:
+// dst[map[c]*Wd + ld] = src[c*Ws + ls].
+__global__ __launch_bounds__(kBlock) void k_lane_move_map(const float* __restrict__ src, int Ws,
+                                                          int ls, float* __restrict__ dst, int Wd,
+                                                          int ld, const uint32_t* __restrict__ map,
+                                                          uint32_t n) {
+  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+  if (c < n) dst[(size_t)map[c] * Wd + ld] = src[(size_t)c * Ws + ls];
 }
 
 __global__ __launch_bounds__(kBlock) void k_lane_move_i32(const int32_t* __restrict__ src, int Ws,
@@ -272,6 +351,13 @@ struct mlem_batch {
   mlem_matrix* mat = nullptr;
   uint32_t F = 0;
   int wmax = 32;
+  // MLEM_MODE_FAST: blocks of width > 1 run through the matrix's tiling P,
+  // their columns in panel order, partial sums in fpart / upart (sized for
+  // wmax); MLEM_MODE_EXACT: per-frame sequential sums over the CSR / CSC
+  int mode = MLEM_MODE_FAST;
+  const PanelFormat* P = nullptr;
+  float* fpart = nullptr;
+  float* upart = nullptr;
   // measured cost (ms) of one block-iteration at widths 4, 8, 16, 32
   // (autotuned batches; 0 = not measured)
   double wcost[4] = {0, 0, 0, 0};
@@ -303,6 +389,8 @@ struct mlem_batch {
     dfree(y_file);
     dfree(tiles_f);
     dfree(tiles_b);
+    dfree(fpart);
+    dfree(upart);
   }
 };
 
@@ -323,7 +411,7 @@ std::unique_ptr<RhsBlock> new_block(mlem_batch* B, int W) {
   MLEM_HIP(hipMalloc(&b->X, 4ull * m.cols * W));
   MLEM_HIP(hipMalloc(&b->Y, 4ull * m.rows * W));
   MLEM_HIP(hipMalloc(&b->R, 4ull * m.rows * W));
-  if (W > 1) MLEM_HIP(hipMalloc(&b->F, 4ull * m.rows * W));
+  if (W > 1 && B->mode == MLEM_MODE_EXACT) MLEM_HIP(hipMalloc(&b->F, 4ull * m.rows * W));
   MLEM_HIP(hipMalloc(&b->mask, 4));
   // width 1 runs the single-image forward kernel, whose grid follows the
   // matrix's launch configuration
@@ -370,8 +458,33 @@ void upload_mask(mlem_batch* B, RhsBlock& b) {
   MLEM_HIP(hipStreamSynchronize(B->mat->m.stream));
 }
 
+// Columns of a block's image: panel order for the tiled (FAST) blocks,
+// device order otherwise; the map takes file columns there.
+inline bool tiled_block(const mlem_batch* B, int W) { return B->mode == MLEM_MODE_FAST && W > 1; }
+inline const uint32_t* file_to_block(const mlem_batch* B, int W) {
+  return tiled_block(B, W) ? B->P->pc_of_file : B->mat->m.colmap;
+}
+
+template <int W>
+void launch_tiled_block(mlem_batch* B, RhsBlock& b, hipStream_t s) {
+  const DevMatrix& m = B->mat->m;
+  const PanelFormat& P = *B->P;
+  launch_panel_fwd<W>(P, b.X, B->fpart, B->grid, s);
+  hipLaunchKernelGGL((k_tiled_ratio<W>), dim3(B->grid), dim3(kBlock), 0, s, P.rowslot, B->fpart,
+                     b.Y, b.R, m.rows, b.partials);
+  hipLaunchKernelGGL((k_batch_stats_reduce<W>), dim3(1), dim3(kReduceLanes), 0, s, b.partials,
+                     B->grid, b.counter, b.results, kRing);
+  launch_panel_bwd<W>(P, b.R, B->upart, B->grid, s);
+  hipLaunchKernelGGL((k_tiled_update<W>), dim3(B->grid), dim3(kBlock), 0, s, P.colslot, B->upart,
+                     P.norm, b.X, b.mask, m.cols);
+}
+
 template <int W>
 void launch_block(mlem_batch* B, RhsBlock& b, hipStream_t s) {
+  if (tiled_block(B, W)) {
+    launch_tiled_block<W>(B, b, s);
+    return;
+  }
   const DevMatrix& m = B->mat->m;
   hipLaunchKernelGGL((k_fwd_batch<W>), dim3(B->grid), dim3(kBlock), 0, s, m.rowptr, m.csr,
                      B->tiles_f, B->nt_f, b.X, b.Y, b.R, b.F);
@@ -395,8 +508,8 @@ void launch_single(mlem_batch* B, RhsBlock& b, hipStream_t s) {
   st.results = b.results;
   st.ring = kRing;
   st.max_blocks = b.pmax;
-  launch_fwd(m, MLEM_MODE_EXACT, b.X, b.Y, b.R, true, &st, s);
-  launch_bwd(m, MLEM_MODE_EXACT, b.R, m.norm, b.X, true, s);
+  launch_fwd(m, B->mode, b.X, b.Y, b.R, true, &st, s);
+  launch_bwd(m, B->mode, b.R, m.norm, b.X, true, s);
 }
 
 void enqueue_iteration(mlem_batch* B, hipStream_t s) {
@@ -522,7 +635,8 @@ void repack(mlem_batch* B) {
       const int f = b->frames[l];
       if (f >= 0 && !B->active[f]) {
         hipLaunchKernelGGL(k_lane_out, dim3(blocks_for(m.cols)), dim3(kBlock), 0, m.stream, b->X,
-                           b->W, l, B->final_x + (size_t)f * m.cols, m.colmap, m.cols);
+                           b->W, l, B->final_x + (size_t)f * m.cols, file_to_block(B, b->W),
+                           m.cols);
         b->frames[l] = -1;
       }
     }
@@ -565,8 +679,13 @@ void repack(mlem_batch* B) {
       }
       if (bi == nbk.size()) raise(MLEM_ESTATE, "batched frame lost during repacking");
       RhsBlock& d = *nbk[bi];
-      hipLaunchKernelGGL(k_lane_move, dim3(blocks_for(m.cols)), dim3(kBlock), 0, m.stream, src->X,
-                         src->W, ls, d.X, d.W, l, m.cols);
+      if (tiled_block(B, src->W) == tiled_block(B, d.W))
+        hipLaunchKernelGGL(k_lane_move, dim3(blocks_for(m.cols)), dim3(kBlock), 0, m.stream,
+                           src->X, src->W, ls, d.X, d.W, l, m.cols);
+      else  // panel order <-> device order (a tiled block and a lone frame)
+        hipLaunchKernelGGL(k_lane_move_map, dim3(blocks_for(m.cols)), dim3(kBlock), 0, m.stream,
+                           src->X, src->W, ls, d.X, d.W, l,
+                           tiled_block(B, src->W) ? B->P->dev_of_pc : B->P->pc_of_dev, m.cols);
       hipLaunchKernelGGL(k_lane_move_i32, dim3(blocks_for(m.rows)), dim3(kBlock), 0, m.stream,
                          src->Y, src->W, ls, d.Y, d.W, l, m.rows);
       d.frames[l] = f;
@@ -598,6 +717,7 @@ void mlem_batch_options_default(mlem_batch_options* o) {
   o->rel_tol = 0.0;
   o->block_width = 32;
   o->use_graph = 1;
+  o->mode = MLEM_MODE_FAST;
 }
 
 int mlem_batch_create(mlem_matrix* h, uint32_t frames, const int32_t* y, uint32_t rows,
@@ -618,11 +738,14 @@ int mlem_batch_create(mlem_matrix* h, uint32_t frames, const int32_t* y, uint32_
     if (o.block_width != 0 && o.block_width != 1 && o.block_width != 4 && o.block_width != 8 &&
         o.block_width != 16 && o.block_width != 32)
       raise(MLEM_EINVAL, "block_width must be 0 (autotune), 1, 4, 8, 16 or 32");
+    if (o.mode != MLEM_MODE_FAST && o.mode != MLEM_MODE_EXACT)
+      raise(MLEM_EINVAL, "batch mode must be MLEM_MODE_FAST or MLEM_MODE_EXACT");
     const bool autotune = o.block_width == 0;
     std::unique_ptr<mlem_batch> B(new mlem_batch());
     B->mat = h;
     B->F = frames;
     B->wmax = autotune ? 32 : o.block_width;
+    B->mode = o.mode;
     B->tol = o.rel_tol;
     B->use_graph = o.use_graph != 0;
     B->budget.assign(frames, 0x7FFFFFFF);
@@ -638,6 +761,12 @@ int mlem_batch_create(mlem_matrix* h, uint32_t frames, const int32_t* y, uint32_
       if (B->budget[f] == 0) B->active[f] = 0;
     B->grid = (uint32_t)std::max(8, 8 * m.cus);
     B->graph_gen = m.launch_gen;
+    if (B->mode == MLEM_MODE_FAST && B->wmax > 1) {
+      // the tiling (built once per matrix) and one block's partial sums
+      B->P = &ensure_panels(m);
+      MLEM_HIP(hipMalloc(&B->fpart, 4ull * std::max<uint64_t>(B->P->nseg, 1) * B->wmax));
+      MLEM_HIP(hipMalloc(&B->upart, 4ull * std::max<uint64_t>(B->P->ngroups, 1) * B->wmax));
+    }
     // tiles (device-order pointers are on the host after open)
     prepare_tiles(m, m.stream);
     const std::vector<uint32_t> tf = spmm::partition(m.h_rowptr, kTile),
@@ -783,7 +912,7 @@ int mlem_batch_image(mlem_batch* B, uint32_t frame, float* x, uint32_t cols) {
       for (int l = 0; l < b->W; ++l)
         if (b->frames[l] == (int)frame)
           hipLaunchKernelGGL(k_lane_out, dim3(blocks_for(m.cols)), dim3(kBlock), 0, m.stream,
-                             b->X, b->W, l, src, m.colmap, m.cols);
+                             b->X, b->W, l, src, file_to_block(B, b->W), m.cols);
     check_launch("image copy");
     MLEM_HIP(hipMemcpyAsync(x, src, 4ull * cols, hipMemcpyDeviceToHost, m.stream));
     MLEM_HIP(hipStreamSynchronize(m.stream));

@@ -127,7 +127,11 @@ int main(int argc, char* argv[]) {
   mlem_options_default(&o);
   o.mode = mode;
   o.verbose = 1;
-  o.checkpoint_interval = checkpointing;
+  // The 5-argument form restarts from a complete checkpoint even when its
+  // <checkpointing> argument is 0 (mpicsr4mlem.cpp:536-549); it then writes
+  // none. The 4-argument form is seqcsr4mlem, which has no checkpoints.
+  o.checkpoint_interval = argc == 6 && checkpointing == 0 ? MLEM_CHECKPOINT_RESTORE_ONLY
+                                                          : checkpointing;
   int rc = mlem_reconstruct(m, y.data(), rows, image.data(), cols, iterations, &o);
   if (rc != MLEM_OK) {
     const std::string msg = mlem_last_error();

@@ -124,6 +124,36 @@ struct StreamedMatrix {
   ~StreamedMatrix();
 };
 
+// Two-dimensional tiling of the matrix for products with W interleaved
+// right-hand sides (panels.hpp; built on first use by ensure_panels). Vectors
+// of those products index columns in panel order.
+struct PanelFormat {
+  uint32_t npanels = 0, nchunks = 0;
+  uint64_t nseg = 0, ngroups = 0;      // forward segments, backward groups (slots)
+  uint32_t* pc_of_file = nullptr;      // [cols] panel-order column of each file column
+  uint32_t* pc_of_dev = nullptr;       // [cols] ... of each device column
+  uint32_t* dev_of_pc = nullptr;       // [cols] device column of each panel-order column
+  float* norm = nullptr;               // [cols] column sums in panel order
+  uint32_t* fseg = nullptr;            // [nseg+1] first element of each segment
+  uint32_t* fslot = nullptr;           // [nseg] row-major slot of each segment
+  float* fval = nullptr;               // [nnz]
+  uint16_t* fcol = nullptr;            // [nnz] column inside the panel
+  uint32_t* rowslot = nullptr;         // [rows+1] first slot of each device row
+  uint4* ftiles = nullptr;             // [nftiles+1] {first segment, first element,
+  uint32_t nftiles = 0;                //  first panel-order column of the panel, 0}
+  uint32_t* bseg = nullptr;            // [ngroups+1] first entry of each group
+  uint32_t* bslot = nullptr;           // [ngroups] voxel-major slot of each group
+  uint2* bent = nullptr;               // [nnz] {device row, value bits}
+  uint32_t* colslot = nullptr;         // [cols+1] first slot of each panel-order column
+  uint4* btiles = nullptr;             // [nbtiles+1] {first group, first entry, 0, 0}
+  uint32_t nbtiles = 0;
+  double build_seconds = 0.0;
+  PanelFormat() = default;
+  PanelFormat(const PanelFormat&) = delete;
+  PanelFormat& operator=(const PanelFormat&) = delete;
+  ~PanelFormat();
+};
+
 struct DevMatrix {
   uint32_t rows = 0, cols = 0;
   uint64_t nnz = 0;
@@ -179,6 +209,7 @@ struct DevMatrix {
   float* tf = nullptr;   // [rows] file-order staging
 
   StreamedMatrix* streamed = nullptr;  // out-of-core matrix (no device CSR/CSC)
+  PanelFormat* panels = nullptr;       // tiling for many right-hand sides (or none)
 };
 
 // Deterministic FP64 reduction state for the forward pass: per-block partial
@@ -267,6 +298,20 @@ void launch_quantize(DevMatrix& m, int vk, float* rscale, hipStream_t s);
 void launch_colsums_scaled(const DevMatrix& m, float* norm, hipStream_t s);
 // r_i *= s_i in device row order (no-op for fp32 values).
 void launch_scale_rows(const DevMatrix& m, float* r, hipStream_t s);
+// ---- products with W right-hand sides through the tiling (panel.hip) ------
+// Build (once per matrix, on the host from the device CSR) and return the
+// tiling; free_panels releases it.
+const PanelFormat& ensure_panels(DevMatrix& m);
+void free_panels(DevMatrix& m);
+// part[fslot * W + f] = segment dot products with X (panel order, [cols][W]);
+// part[bslot * W + f] = group dot products with R ([rows][W], device rows).
+// grid: a multiple of 8 workgroups.
+template <int W>
+void launch_panel_fwd(const PanelFormat& P, const float* X, float* part, uint32_t grid,
+                      hipStream_t s);
+template <int W>
+void launch_panel_bwd(const PanelFormat& P, const float* R, float* part, uint32_t grid,
+                      hipStream_t s);
 // ---- load-time layout (layout.hip) ----------------------------------------
 // Renumber rows (sinogram order) and voxels (4x4x2 bricks) when the grid is
 // known, build the CSC (file row order inside every column) and the

@@ -169,3 +169,17 @@ def frame_budgets(frames: int, lo: int = 10, hi: int = 100, seed: int = 4998) ->
     """Iteration budget per frame, uniform integers in [lo, hi]."""
     u = uniform(seed, 0, frames)
     return (lo + np.floor(u * (hi - lo + 1))).astype(np.int32)
+
+
+def dynamic_frames(problem: Problem, frames: int, lo: float = 1e5, hi: float = 1e8) -> np.ndarray:
+    """Sinograms of a dynamic series on a workload's matrix: frame f is the
+    workload's phantom with per-ellipsoid intensity scales U(0.2, 2) (seed
+    4000+f), projected and Poisson-sampled (seed 5000+f) at a total count
+    log-uniform in [lo, hi]. int32 [frames, rows]."""
+    w = WORKLOADS[problem.name]
+    counts = frame_counts(frames, lo, hi)
+    Y = np.empty((frames, problem.matrix.rows), np.int32)
+    for f, sc in enumerate(frame_scales(frames, w["ellipsoids"])):
+        x = make_phantom(w["grid"], w["ellipsoids"], 2000 + w["key"], w["semi"], scale=sc)
+        Y[f] = poisson(project(problem.matrix, x), counts[f], 5000 + f)
+    return Y

@@ -1,8 +1,10 @@
 """Batched (many-sinogram) reconstruction on the GPU against the oracle.
 
-Every frame's image after k iterations must be bit-identical to k iterations of
-the fp32 restatement of seqcsr4mlem; the per-frame FP64 log-likelihood history
-must match the FP64 oracle's.
+EXACT mode: every frame's image after k iterations must be bit-identical to k
+iterations of the fp32 restatement of seqcsr4mlem. FAST mode (the matrix's
+two-dimensional tiling): every frame within 1e-4 relative of k iterations of
+FP64 MLEM, deterministic run to run. Both: the per-frame FP64 log-likelihood
+history within 1e-6 of the FP64 oracle's.
 """
 from __future__ import annotations
 
@@ -45,7 +47,7 @@ def test_batch_budgets_bitwise(tiny_batch, width):
     p, A, Y = tiny_batch
     m = p.matrix
     budgets = np.array([1, 2, 3, 5, 8, 13, 4, 0, 7, 9, 6], np.int32)
-    B = M.Batch(A, Y, budgets, block_width=width)
+    B = M.Batch(A, Y, budgets, block_width=width, mode="exact")
     n = B.run(100)
     assert n == budgets.max()
     prog = B.progress()
@@ -66,7 +68,8 @@ def test_batch_early_stop(tiny_batch, width):
     p, A, Y = tiny_batch
     m = p.matrix
     tol = 2e-4
-    B = M.Batch(A, Y, np.full(Y.shape[0], 60, np.int32), rel_tol=tol, block_width=width)
+    B = M.Batch(A, Y, np.full(Y.shape[0], 60, np.int32), rel_tol=tol, block_width=width,
+                mode="exact")
     B.run(1000)
     for f in range(Y.shape[0]):
         it, ll = B.frame_info(f)
@@ -82,8 +85,8 @@ def test_batch_early_stop(tiny_batch, width):
 def test_batch_graph_matches_direct(tiny_batch):
     p, A, Y = tiny_batch
     budgets = np.array([3, 6, 2, 5, 4, 1, 6, 6, 3, 2, 5], np.int32)
-    a = M.Batch(A, Y, budgets, block_width=4, use_graph=True)
-    b = M.Batch(A, Y, budgets, block_width=4, use_graph=False)
+    a = M.Batch(A, Y, budgets, block_width=4, use_graph=True, mode="exact")
+    b = M.Batch(A, Y, budgets, block_width=4, use_graph=False, mode="exact")
     a.run(10)
     b.run(10)
     for f in range(Y.shape[0]):
@@ -95,7 +98,7 @@ def test_batch_graph_matches_direct(tiny_batch):
 def test_batch_incremental_runs(tiny_batch):
     p, A, Y = tiny_batch
     m = p.matrix
-    B = M.Batch(A, Y[:5], None, block_width=4)
+    B = M.Batch(A, Y[:5], None, block_width=4, mode="exact")
     assert B.run(3) == 3
     assert B.run(4) == 4
     for f in range(5):
@@ -113,7 +116,7 @@ def test_batch_small_workload(width):
     A = M.Csr4Matrix.from_arrays(m.rows, m.cols, m.rowidx, m.elements, grid=m.grid)
     Y = make_frames(p, 6, 1e5, 1e7)
     budgets = np.array([12, 3, 7, 1, 10, 5], np.int32)
-    B = M.Batch(A, Y, budgets, block_width=width)
+    B = M.Batch(A, Y, budgets, block_width=width, mode="exact")
     B.run(50)
     for f in range(6):
         xo = orc.mlem(m.rowidx, m.elements, m.cols, Y[f], int(budgets[f]))[0]
@@ -125,11 +128,15 @@ def test_batch_small_workload(width):
 def test_batch_argument_errors(tiny_batch):
     p, A, Y = tiny_batch
     with pytest.raises(ValueError):
-        M.Batch(A, Y[:, :-1])
+        M.Batch(A, Y[:, :-1], mode="exact")
     with pytest.raises(M.MlemError):
-        M.Batch(A, Y, block_width=12)
+        M.Batch(A, Y, block_width=12, mode="exact")
     with pytest.raises(M.MlemError):
-        M.Batch(A, Y, np.full(Y.shape[0], -1, np.int32))
+        M.Batch(A, Y, np.full(Y.shape[0], -1, np.int32), mode="exact")
+    with pytest.raises(ValueError):   # one budget per frame
+        M.Batch(A, Y, np.full(Y.shape[0] - 1, 3, np.int32))
+    with pytest.raises(KeyError):
+        M.Batch(A, Y, mode="atomic-ish")
 
 
 def cheapest_blocks(ms, frames):
@@ -152,7 +159,7 @@ def cheapest_blocks(ms, frames):
 
 def test_batch_autotuned_widths(tiny_batch):
     p, A, Y = tiny_batch
-    B = M.Batch(A, Y, np.full(Y.shape[0], 3, np.int32), block_width=0)
+    B = M.Batch(A, Y, np.full(Y.shape[0], 3, np.int32), block_width=0, mode="exact")
     ms = B.tuning()
     assert set(ms) == {1, 4, 8, 16, 32} and all(v > 0 for v in ms.values())
     # the chosen blocks are the cheapest set for the measured costs; no
@@ -165,7 +172,8 @@ def test_batch_autotuned_widths(tiny_batch):
     m = p.matrix
     for f in range(Y.shape[0]):
         assert np.array_equal(B.image(f), orc.mlem(m.rowidx, m.elements, m.cols, Y[f], 3)[0])
-    assert M.Batch(A, Y, block_width=32).tuning() == {1: 0.0, 4: 0.0, 8: 0.0, 16: 0.0, 32: 0.0}
+    assert M.Batch(A, Y, block_width=32, mode="exact").tuning() == {1: 0.0, 4: 0.0, 8: 0.0,
+                                                                     16: 0.0, 32: 0.0}
 
 
 def test_batch_single_frame_blocks_after_launch_change(tiny_batch):
@@ -174,7 +182,7 @@ def test_batch_single_frame_blocks_after_launch_change(tiny_batch):
     p, A, Y = tiny_batch
     m = p.matrix
     budgets = np.array([2, 7, 4, 1, 6, 3, 5, 7, 2, 6, 4], np.int32)
-    B = M.Batch(A, Y, budgets, block_width=1)
+    B = M.Batch(A, Y, budgets, block_width=1, mode="exact")
     assert B.progress()["blocks"] == int(np.count_nonzero(budgets))
     B.run(2)
     A.set_launch(("dict", 16), ("tiles", 8))
@@ -188,3 +196,109 @@ def test_batch_single_frame_blocks_after_launch_change(tiny_batch):
         xo = orc.mlem(m.rowidx, m.elements, m.cols, Y[f], int(budgets[f]))[0]
         assert np.array_equal(B.image(f), xo), f
     B.close()
+
+
+# ---- FAST mode: the matrix's two-dimensional tiling --------------------------
+TOL_FAST = 1e-4   # relative L-infinity per voxel vs FP64 MLEM (floor 1e-6 max)
+
+
+def _vs_fp64(m, B, Y, frames=None):
+    """Largest relative distance of the frames' images to FP64 MLEM after the
+    same number of iterations, and of their log-likelihood histories."""
+    worst, worst_ll = 0.0, 0.0
+    for f in (range(Y.shape[0]) if frames is None else frames):
+        it, ll = B.frame_info(f)
+        x64, ll64, _ = orc.f64_mlem(m.rowidx, m.elements, m.cols, Y[f].astype(np.float64), it)
+        worst = max(worst, rel_linf(B.image(f), x64))
+        if it:
+            worst_ll = max(worst_ll, rel_linf(ll, ll64))
+    return worst, worst_ll
+
+
+@pytest.mark.parametrize("width", [1, 4, 8, 16, 32, 0])
+def test_fast_batch_budgets_vs_fp64(tiny_batch, width):
+    p, A, Y = tiny_batch
+    budgets = np.array([1, 2, 3, 5, 8, 13, 4, 0, 7, 9, 6], np.int32)
+    B = M.Batch(A, Y, budgets, block_width=width, mode="fast")
+    assert B.run(100) == budgets.max()
+    assert B.progress()["frame_iterations"] == budgets.sum()
+    for f in range(Y.shape[0]):
+        assert B.frame_info(f)[0] == budgets[f]
+    e, ell = _vs_fp64(p.matrix, B, Y)
+    print(f"[fast batch] width {width}: rel Linf vs FP64 {e:.2e}, loglik {ell:.2e}")
+    assert e < TOL_FAST and ell < 1e-6
+    B.close()
+
+
+def test_fast_batch_deterministic_graph_and_runs(tiny_batch):
+    # the tiled sums are in a fixed order: the same bits run to run, with or
+    # without graph replay, in one run or in several
+    p, A, Y = tiny_batch
+    budgets = np.array([3, 6, 2, 5, 4, 1, 6, 6, 3, 2, 5], np.int32)
+    a = M.Batch(A, Y, budgets, block_width=8, use_graph=True)
+    b = M.Batch(A, Y, budgets, block_width=8, use_graph=False)
+    c = M.Batch(A, Y, budgets, block_width=8, use_graph=True)
+    a.run(10)
+    b.run(10)
+    c.run(2)
+    c.run(3)
+    c.run(10)
+    for f in range(Y.shape[0]):
+        assert np.array_equal(a.image(f), b.image(f)), f
+        assert np.array_equal(a.image(f), c.image(f)), f
+        assert np.array_equal(a.frame_info(f)[1], c.frame_info(f)[1]), f
+    for B in (a, b, c):
+        B.close()
+
+
+@pytest.mark.parametrize("width", [1, 8])
+def test_fast_batch_early_stop(tiny_batch, width):
+    p, A, Y = tiny_batch
+    tol = 2e-4
+    B = M.Batch(A, Y, np.full(Y.shape[0], 60, np.int32), rel_tol=tol, block_width=width)
+    B.run(1000)
+    for f in range(Y.shape[0]):
+        it, ll = B.frame_info(f)
+        assert 2 <= it <= 60
+        rel = np.abs(np.diff(ll)) / np.abs(ll[1:])
+        if it < 60:
+            assert rel[-1] <= tol and np.all(rel[:-1] > tol)
+    e, ell = _vs_fp64(p.matrix, B, Y)
+    assert e < TOL_FAST and ell < 1e-6
+    B.close()
+
+
+@pytest.mark.parametrize("width", [4, 16, 0])
+def test_fast_batch_small_workload(width):
+    # 32^3 grid, 262,144 lines (two panels): 9 frames, budgets up to 12;
+    # autotuned widths mix tiled blocks and lone frames on the single-image
+    # FAST kernels, so frames move between the two column orders
+    p = synth.make_problem("small")
+    m = p.matrix
+    A = M.Csr4Matrix.from_arrays(m.rows, m.cols, m.rowidx, m.elements, grid=m.grid)
+    Y = make_frames(p, 9, 1e5, 1e7)
+    budgets = np.array([12, 3, 7, 1, 10, 5, 12, 2, 9], np.int32)
+    B = M.Batch(A, Y, budgets, block_width=width)
+    B.run(50)
+    n = M.calcColumnSums(A).astype(np.float64)
+    for f in range(Y.shape[0]):
+        ysum = float(Y[f].astype(np.float64).sum())
+        assert abs(float(n @ B.image(f).astype(np.float64)) - ysum) <= 1e-5 * ysum
+    e, ell = _vs_fp64(m, B, Y)
+    print(f"[fast batch] small, width {width}: rel Linf vs FP64 {e:.2e}, loglik {ell:.2e}")
+    assert e < TOL_FAST and ell < 1e-6
+    B.close()
+    A.close()
+
+
+def test_fast_batch_file_order_matrix(tiny_problem):
+    # a matrix opened without a grid: panels are runs of columns
+    m = tiny_problem.matrix
+    A = M.Csr4Matrix.from_arrays(m.rows, m.cols, m.rowidx, m.elements)
+    Y = make_frames(tiny_problem, 5)
+    B = M.Batch(A, Y, np.array([4, 2, 6, 1, 3], np.int32), block_width=4)
+    B.run(10)
+    e, ell = _vs_fp64(m, B, Y)
+    assert e < TOL_FAST and ell < 1e-6
+    B.close()
+    A.close()

@@ -24,9 +24,9 @@ TOL_PROJ = 2e-6   # FAST projections vs FP64, elementwise relative
 TOL_RECON = 1e-4  # FAST reconstructions vs FP64, elementwise relative
 
 
-def _open(p):
+def _open(p, **kw):
     m = p.matrix
-    return M.Csr4Matrix.from_arrays(m.rows, m.cols, m.rowidx, m.elements, grid=m.grid)
+    return M.Csr4Matrix.from_arrays(m.rows, m.cols, m.rowidx, m.elements, grid=m.grid, **kw)
 
 
 @pytest.fixture(scope="module")
@@ -120,7 +120,7 @@ def test_pet_batched_frames_bitwise(pet):
     m = p.matrix
     Y = np.stack([p.y, np.maximum(p.y // 3, 0).astype(np.int32)])
     budgets = np.array([3, 2], np.int32)
-    B = M.Batch(A, Y, budgets, block_width=4)
+    B = M.Batch(A, Y, budgets, block_width=4, mode="exact")
     assert B.run(10) == 3
     for f in range(2):
         xo = orc.mlem(m.rowidx, m.elements, m.cols, Y[f], int(budgets[f]))[0]
@@ -136,7 +136,7 @@ def test_pet_batched_autotuned_lone_frames(pet):
     m = p.matrix
     Y = np.stack([p.y, (p.y // 2).astype(np.int32), (p.y // 5).astype(np.int32)])
     budgets = np.array([3, 2, 4], np.int32)
-    B = M.Batch(A, Y, budgets, block_width=0)
+    B = M.Batch(A, Y, budgets, block_width=0, mode="exact")
     ms = B.tuning()
     assert 3 * ms[1] < ms[4]
     prog = B.progress()
@@ -175,3 +175,126 @@ def test_large_projections_and_counts(large):
     n = M.calcColumnSums(A).astype(np.float64)
     ysum = float(p.y.astype(np.float64).sum())
     assert abs(float(n @ xk.astype(np.float64)) - ysum) <= 1e-5 * ysum
+
+
+# ---- batched frames at the size of the 64-frame dynamic workload ------------
+# 64 sinograms on the pet matrix (per-ellipsoid intensity scales U(0.2, 2),
+# counts log-uniform in 1e5..1e8). Per frame the batched arithmetic is that of
+# mlem() (seqcsr4mlem.cpp:137-189): checked bit for bit on three frames against
+# the fp32 restatement, and on every frame through count preservation
+# sum_j n_j x_j = sum_i y_i and a non-decreasing FP64 log-likelihood.
+
+@pytest.fixture(scope="module")
+def pet_frames(pet):
+    p, _ = pet
+    return synth.dynamic_frames(p, 64)
+
+
+def _check_frames(A, Y, B, budgets=None):
+    n = M.calcColumnSums(A).astype(np.float64)
+    for f in range(Y.shape[0]):
+        it, ll = B.frame_info(f)
+        if budgets is not None:
+            assert it <= budgets[f]
+        if it == 0:
+            continue
+        ysum = float(Y[f].astype(np.float64).sum())
+        x = B.image(f).astype(np.float64)
+        assert np.all(x >= 0)
+        assert abs(float(n @ x) - ysum) <= 1e-5 * ysum, f
+        # EM never lowers the likelihood; ll[k] belongs to the image entering
+        # iteration k (FP64 sums of fp32 projections: 1e-12 relative slack)
+        assert np.all(np.diff(ll) >= -1e-12 * np.abs(ll[1:])), f
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+@pytest.mark.parametrize("width", [8, 16, 32])
+def test_pet_64_frames_explicit_width(pet, pet_frames, width, mode):
+    # EXACT: three frames bit for bit against the fp32 restatement; FAST (the
+    # tiled product): the same frames within 1e-4 of FP64 MLEM
+    p, A = pet
+    m = p.matrix
+    Y = pet_frames
+    budgets = (1 + np.arange(Y.shape[0]) % 5).astype(np.int32)   # 1..5 iterations
+    B = M.Batch(A, Y, budgets, block_width=width, use_graph=True, mode=mode)
+    assert B.progress()["block_width"] == width
+    assert B.run(100) == 5
+    assert B.progress()["frame_iterations"] == budgets.sum()
+    _check_frames(A, Y, B, budgets)
+    for f in (4, 33, 62):                                        # 5, 4 and 3 iterations
+        k = int(budgets[f])
+        if mode == "exact":
+            xo = orc.mlem(m.rowidx, m.elements, m.cols, Y[f], k)[0]
+            assert np.array_equal(B.image(f), xo), f
+        else:
+            x64 = orc.f64_mlem_par(m.rowidx, m.elements, m.cols, Y[f].astype(np.float64), k)[0]
+            assert rel_linf(B.image(f), x64) < TOL_RECON, f
+    B.close()
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_pet_64_frame_dynamic_workload(pet, pet_frames, mode):
+    # the benchmark's batched workload: budgets 10..100, early stop at a
+    # relative log-likelihood change of 1e-6, autotuned block widths,
+    # graph-replayed iterations
+    p, A = pet
+    m = p.matrix
+    Y = pet_frames
+    budgets = synth.frame_budgets(Y.shape[0])
+    tol = 1e-6
+    B = M.Batch(A, Y, budgets, rel_tol=tol, block_width=0, use_graph=True, mode=mode)
+    B.run(1000)
+    assert B.progress()["active"] == 0
+    _check_frames(A, Y, B, budgets)
+    its = np.array([B.frame_info(f)[0] for f in range(Y.shape[0])])
+    for f in range(Y.shape[0]):
+        it, ll = B.frame_info(f)
+        if it < budgets[f]:   # stopped early: at the first change below the tolerance
+            rel = np.abs(np.diff(ll)) / np.abs(ll[1:])
+            assert rel[-1] <= tol and np.all(rel[:-1] > tol), f
+    f = int(np.argmin(its))
+    if mode == "exact":
+        xo = orc.mlem(m.rowidx, m.elements, m.cols, Y[f], int(its[f]))[0]
+        assert np.array_equal(B.image(f), xo)
+    x64, ll64, _ = orc.f64_mlem_par(m.rowidx, m.elements, m.cols, Y[f].astype(np.float64),
+                                    int(its[f]))
+    assert rel_linf(B.image(f), x64) < TOL_RECON
+    assert rel_linf(B.frame_info(f)[1], ll64) < 1e-6
+    B.close()
+
+
+# ---- 16-bit matrix values on the 192x192x64 / 4,194,304-line matrix ---------
+
+@pytest.mark.parametrize("kind", ["fp16", "bf16"])
+def test_large_16bit_values(large, kind):
+    # quantized at open (row scale s_i = max_k |a_ik|, q_ij = a_ij / s_i in
+    # 16 bits): EXACT projections equal the fp32 restatement's projections of
+    # the oracle's quantized matrix bit for bit; FAST projections within 2e-6
+    # of FP64 on s_i q_ij and adjoint to each other; counts preserved after
+    # two FAST iterations with the FP64 column sums of s_i q_ij
+    p, _ = large
+    m = p.matrix
+    A = _open(p, precision=kind)
+    try:
+        q, s = orc.quantize(m.rowidx, m.elements, kind)
+        rng = np.random.default_rng(13)
+        x = rng.uniform(0.1, 2.0, m.cols).astype(np.float32)
+        r = rng.uniform(0.1, 2.0, m.rows).astype(np.float32)
+        assert np.array_equal(M.calcFwProj(A, x, "exact"), orc.fwproj(m.rowidx, q, x) * s)
+        assert np.array_equal(M.calcBkProj(A, r, "exact"), orc.bkproj(m.rowidx, q, r * s, m.cols))
+        s64 = s.astype(np.float64)
+        f = M.calcFwProj(A, x, "fast")
+        u = M.calcBkProj(A, r, "fast")
+        assert rel_linf(f, orc.f64_fwproj(m.rowidx, q, x) * s64) < TOL_PROJ
+        assert rel_linf(u, orc.f64_bkproj(m.rowidx, q, r.astype(np.float64) * s64,
+                                          m.cols)) < TOL_PROJ
+        lhs = float(f.astype(np.float64) @ r.astype(np.float64))
+        rhs = float(x.astype(np.float64) @ u.astype(np.float64))
+        assert abs(lhs - rhs) <= 1e-6 * abs(lhs)
+        del q
+        xk = M.mlem(A, p.y, None, 2, mode="fast")
+        n = M.calcColumnSums(A).astype(np.float64)
+        ysum = float(p.y.astype(np.float64).sum())
+        assert abs(float(n @ xk.astype(np.float64)) - ysum) <= 1e-5 * ysum
+    finally:
+        A.close()
