@@ -185,41 +185,95 @@ __global__ __launch_bounds__(kBlock) void k_bwd_batch(
   spmm::walk<W>(colptr, csc, tiles, ntiles, R, epi);
 }
 
-// Epilogue of the tiled forward product (FAST mode): lane f of row group g
-// adds the row's segment slots in panel order, then ratio r = (f != 0) ? y/f
-// : 0 and the frame's FP64 statistics over rows g, g + groups, ...; one
-// partial per workgroup and frame.
+// The tiled (FAST) epilogues give each lane four right-hand sides (16-byte
+// loads and stores): W/4 lanes per row or voxel.
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_tiled_ratio(const uint32_t* __restrict__ rowslot,
+struct Quads {
+  static_assert(W % 4 == 0, "right-hand sides come in quads");
+  static constexpr uint32_t L = W / 4;
+  static constexpr uint32_t G = kBlock / L;
+};
+
+// batch_stats_partial for lanes that carry four frames each (lane q of a
+// group holds frames 4q .. 4q+3): one partial per workgroup and frame, the
+// groups summed in a fixed order.
+template <int W>
+__device__ void batch_stats_partial4(const double (&ll)[4], const double (&tot)[4],
+                                     double2* __restrict__ partials) {
+  constexpr uint32_t L = Quads<W>::L, G = Quads<W>::G;
+  __shared__ double s_ll[4 * kBlock], s_tot[4 * kBlock];
+  const uint32_t t = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    s_ll[4 * t + j] = ll[j];
+    s_tot[4 * t + j] = tot[j];
+  }
+  __syncthreads();
+  if (t < (uint32_t)W) {
+    double a = 0.0, b = 0.0;
+    for (uint32_t g = 0; g < G; ++g) {
+      const uint32_t i = 4 * (g * L + t / 4) + t % 4;
+      a += s_ll[i];
+      b += s_tot[i];
+    }
+    partials[(size_t)blockIdx.x * W + t] = make_double2(a, b);
+  }
+}
+
+__device__ __forceinline__ void add4(float4& a, const float4& b) {
+  a.x += b.x;
+  a.y += b.y;
+  a.z += b.z;
+  a.w += b.w;
+}
+
+__device__ __forceinline__ float ratio1(int32_t y, float f) { return (f != 0.f) ? (float)y / f : 0.f; }
+
+__device__ __forceinline__ void stat1(int32_t y, float f, double& ll, double& tot) {
+  tot += (double)f;
+  if (f > 0.f) ll += (y != 0 ? (double)y * log((double)f) : 0.0) - (double)f;
+}
+
+// Epilogue of the tiled forward product (FAST mode): a row's segment slots
+// added in panel order, then ratio r = (f != 0) ? y/f : 0 and the frames'
+// FP64 statistics (row groups g, g + groups, ...); one partial per workgroup
+// and frame.
+// (64 VGPRs: the grid of 8 workgroups per CU is resident at once)
+template <int W>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_tiled_ratio(const uint32_t* __restrict__ rowslot,
                                                         const float* __restrict__ part,
                                                         const int32_t* __restrict__ Y,
                                                         float* __restrict__ R, uint32_t rows,
                                                         double2* __restrict__ partials) {
-  constexpr uint32_t G = kBlock / W;
-  const uint32_t f = threadIdx.x % W, g = threadIdx.x / W;
+  constexpr uint32_t L = Quads<W>::L, G = Quads<W>::G;
+  const uint32_t q = threadIdx.x % L, g = threadIdx.x / L;
   const uint32_t stride = gridDim.x * G;
-  double ll = 0.0, tot = 0.0;
+  const float4* __restrict__ p4 = reinterpret_cast<const float4*>(part);
+  const int4* __restrict__ y4 = reinterpret_cast<const int4*>(Y);
+  float4* __restrict__ r4 = reinterpret_cast<float4*>(R);
+  double ll[4] = {0.0, 0.0, 0.0, 0.0}, tot[4] = {0.0, 0.0, 0.0, 0.0};
   for (uint32_t row = blockIdx.x * G + g; row < rows; row += stride) {
     uint32_t s = rowslot[row];
     const uint32_t e = rowslot[row + 1];
-    float F = 0.f;
-    for (; s + 4 <= e; s += 4) {
-      const float a0 = part[(size_t)s * W + f], a1 = part[(size_t)(s + 1) * W + f];
-      const float a2 = part[(size_t)(s + 2) * W + f], a3 = part[(size_t)(s + 3) * W + f];
-      F += a0;
-      F += a1;
-      F += a2;
-      F += a3;
+    float4 F = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (; s + 2 <= e; s += 2) {
+      const float4 a0 = p4[(size_t)s * L + q], a1 = p4[(size_t)(s + 1) * L + q];
+      add4(F, a0);
+      add4(F, a1);
     }
-    for (; s < e; ++s) F += part[(size_t)s * W + f];
-    const size_t i = (size_t)row * W + f;
-    const int32_t yi = Y[i];
-    R[i] = (F != 0.f) ? (float)yi / F : 0.f;
-    tot += (double)F;
-    if (F > 0.f) ll += (yi != 0 ? (double)yi * log((double)F) : 0.0) - (double)F;
+    if (s < e) add4(F, p4[(size_t)s * L + q]);
+    const size_t i = (size_t)row * L + q;
+    const int4 y = y4[i];
+    r4[i] = make_float4(ratio1(y.x, F.x), ratio1(y.y, F.y), ratio1(y.z, F.z), ratio1(y.w, F.w));
+    stat1(y.x, F.x, ll[0], tot[0]);
+    stat1(y.y, F.y, ll[1], tot[1]);
+    stat1(y.z, F.z, ll[2], tot[2]);
+    stat1(y.w, F.w, ll[3], tot[3]);
   }
-  batch_stats_partial<W>(ll, tot, partials);
+  batch_stats_partial4<W>(ll, tot, partials);
 }
+
+__device__ __forceinline__ float update1(float x, float u, float n) { return x * ((n != 0.f) ? u / n : u); }
 
 // Epilogue of the tiled backprojection (FAST mode): u = the voxel's group
 // slots added in chunk order, then x *= (n != 0) ? u/n : u for live lanes.
@@ -231,26 +285,34 @@ __global__ __launch_bounds__(kBlock) void k_tiled_update(const uint32_t* __restr
                                                          float* __restrict__ X,
                                                          const uint32_t* __restrict__ mask,
                                                          uint32_t cols) {
-  constexpr uint32_t G = kBlock / W;
-  const uint32_t f = threadIdx.x % W, g = threadIdx.x / W;
-  if (!((*mask >> f) & 1u)) return;
+  constexpr uint32_t L = Quads<W>::L, G = Quads<W>::G;
+  const uint32_t q = threadIdx.x % L, g = threadIdx.x / L;
+  const uint32_t live = (*mask >> (4 * q)) & 0xFu;
+  if (!live) return;
   const uint32_t stride = gridDim.x * G;
+  const float4* __restrict__ p4 = reinterpret_cast<const float4*>(part);
+  float4* __restrict__ x4 = reinterpret_cast<float4*>(X);
   for (uint32_t c = blockIdx.x * G + g; c < cols; c += stride) {
     uint32_t s = colslot[c];
     const uint32_t e = colslot[c + 1];
-    float u = 0.f;
+    float4 u = make_float4(0.f, 0.f, 0.f, 0.f);
     for (; s + 4 <= e; s += 4) {
-      const float a0 = part[(size_t)s * W + f], a1 = part[(size_t)(s + 1) * W + f];
-      const float a2 = part[(size_t)(s + 2) * W + f], a3 = part[(size_t)(s + 3) * W + f];
-      u += a0;
-      u += a1;
-      u += a2;
-      u += a3;
+      const float4 a0 = p4[(size_t)s * L + q], a1 = p4[(size_t)(s + 1) * L + q];
+      const float4 a2 = p4[(size_t)(s + 2) * L + q], a3 = p4[(size_t)(s + 3) * L + q];
+      add4(u, a0);
+      add4(u, a1);
+      add4(u, a2);
+      add4(u, a3);
     }
-    for (; s < e; ++s) u += part[(size_t)s * W + f];
+    for (; s < e; ++s) add4(u, p4[(size_t)s * L + q]);
     const float n = norm[c];
-    const size_t i = (size_t)c * W + f;
-    X[i] = X[i] * ((n != 0.f) ? u / n : u);
+    const size_t i = (size_t)c * L + q;
+    float4 x = x4[i];
+    if (live & 1u) x.x = update1(x.x, u.x, n);
+    if (live & 2u) x.y = update1(x.y, u.y, n);
+    if (live & 4u) x.z = update1(x.z, u.z, n);
+    if (live & 8u) x.w = update1(x.w, u.w, n);
+    x4[i] = x;
   }
 }
 

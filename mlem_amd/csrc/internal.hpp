@@ -128,6 +128,7 @@ struct StreamedMatrix {
 // right-hand sides (panels.hpp; built on first use by ensure_panels). Vectors
 // of those products index columns in panel order.
 struct PanelFormat {
+  uint32_t rows = 0, cols = 0;
   uint32_t npanels = 0, nchunks = 0;
   uint64_t nseg = 0, ngroups = 0;      // forward segments, backward groups (slots)
   uint32_t* pc_of_file = nullptr;      // [cols] panel-order column of each file column

@@ -32,7 +32,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr uint32_t kPTile = 2048;  // elements staged per tile
 constexpr uint32_t kChunkRows = 16384;
-constexpr int kUnroll = 16;  // gathers in flight per lane
+constexpr int kUnroll = 8;  // 16-byte gathers in flight per lane
 
 void check_launch(const char* what) {
   hipError_t e = hipGetLastError();
@@ -68,40 +68,81 @@ void add_tiles(const std::vector<uint32_t>& off, uint32_t s, uint32_t e, uint32_
   }
 }
 
-// Dot product of one staged segment (elements [b, e) of the tile) with lane
-// f's vector: U gathers issued together, then fused multiply-adds in element
-// order. vals / cols may be LDS (staged tile) or global (long segment).
+// A lane carries four right-hand sides (one 16-byte gather per element): the
+// address unit handles a fixed number of lanes per clock, so 16-byte lanes
+// move four times the bytes of 4-byte ones. W/4 lanes share a segment.
+template <int W>
+struct Quad {
+  static_assert(W % 4 == 0, "right-hand sides come in quads");
+  static constexpr uint32_t L = W / 4;        // lanes per segment
+  static constexpr uint32_t G = kBlock / L;   // segments in flight per workgroup
+};
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// 16-byte store that does not allocate in the caches (partial sums are read
+// once, by the next kernel).
+__device__ __forceinline__ void store_nt(float4* p, const float4& v) {
+  const f32x4 w = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(w, reinterpret_cast<f32x4*>(p));
+}
+
+__device__ __forceinline__ uint2 load_nt(const uint2* p) {
+  const u32x2 w = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p));
+  return make_uint2(w.x, w.y);
+}
+
+// Gathers through a buffer resource: one 32-bit offset register per load
+// instead of a 64-bit address (reads past `bytes` return zeros).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                           0x00020000);
+}
+__device__ __forceinline__ float4 gather4(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  const f32x4 v = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
+__device__ __forceinline__ void fma4(float a, const float4& g, float4& acc) {
+  acc.x = fmaf(a, g.x, acc.x);
+  acc.y = fmaf(a, g.y, acc.y);
+  acc.z = fmaf(a, g.z, acc.z);
+  acc.w = fmaf(a, g.w, acc.w);
+}
+
+// Dot products of one segment (elements [b, e) of vals / cols, LDS when the
+// tile is staged, else global) with the lane's four vectors: U gathers
+// issued together, then fused multiply-adds in element order. off0 is the
+// byte offset in X of the lane's quad of the panel's first column; each
+// further column is W*4 bytes on.
 template <int W, int U>
-__device__ __forceinline__ float pdot(const float* vals, const uint16_t* cols, uint32_t b,
-                                      uint32_t e, const float* __restrict__ Xb) {
-  float acc = 0.f;
+__device__ __forceinline__ float4 pdot(const float* vals, const uint16_t* cols, uint32_t b,
+                                       uint32_t e, __amdgpu_buffer_rsrc_t rx, uint32_t off0) {
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   for (uint32_t k = b; k < e; k += U) {
-    float g[U];
+    float4 g[U];
 #pragma unroll
-    for (int j = 0; j < U; ++j) g[j] = Xb[(uint32_t)cols[min(k + j, e - 1)] * W];
+    for (int j = 0; j < U; ++j) g[j] = gather4(rx, off0 + (uint32_t)cols[min(k + j, e - 1)] * (W * 4u));
     // the values are read once the gathers are in flight (fewer live registers)
 #pragma unroll
     for (int j = 0; j < U; ++j)
-      if (k + j < e) acc = fmaf(vals[min(k + j, e - 1)], g[j], acc);
+      if (k + j < e) fma4(vals[min(k + j, e - 1)], g[j], acc);
   }
   return acc;
 }
 
 template <int W, int U>
-__device__ __forceinline__ float rdot(const uint2* ent, uint32_t b, uint32_t e,
-                                      const float* __restrict__ Rf) {
-  float acc = 0.f;
+__device__ __forceinline__ float4 rdot(const uint2* ent, uint32_t b, uint32_t e,
+                                       __amdgpu_buffer_rsrc_t rr, uint32_t off0) {
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   for (uint32_t k = b; k < e; k += U) {
-    float a[U], g[U];
+    float4 g[U];
 #pragma unroll
-    for (int j = 0; j < U; ++j) {
-      const uint2 r = ent[min(k + j, e - 1)];
-      a[j] = __uint_as_float(r.y);
-      g[j] = Rf[(size_t)r.x * W];
-    }
+    for (int j = 0; j < U; ++j) g[j] = gather4(rr, off0 + ent[min(k + j, e - 1)].x * (W * 4u));
 #pragma unroll
     for (int j = 0; j < U; ++j)
-      if (k + j < e) acc = fmaf(a[j], g[j], acc);
+      if (k + j < e) fma4(__uint_as_float(ent[min(k + j, e - 1)].y), g[j], acc);
   }
   return acc;
 }
@@ -114,34 +155,36 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     const uint32_t* __restrict__ fseg, const uint32_t* __restrict__ fslot,
     const float* __restrict__ fval, const uint16_t* __restrict__ fcol,
     const uint4* __restrict__ tiles, uint32_t ntiles, const float* __restrict__ X,
-    float* __restrict__ part) {
-  constexpr uint32_t G = kBlock / W;
+    uint32_t xbytes, float* __restrict__ part) {
+  constexpr uint32_t L = Quad<W>::L, G = Quad<W>::G;
   __shared__ float sval[kPTile];
   __shared__ uint16_t scol[kPTile];
   __shared__ uint32_t S[kBlock + 1];
   __shared__ uint32_t SL[kBlock];
-  const uint32_t t = threadIdx.x, f = t % W, grp = t / W;
+  const uint32_t t = threadIdx.x, q = t % L, grp = t / L;
   const uint32_t xcd = blockIdx.x & 7u, per_xcd = gridDim.x >> 3;
   const uint32_t t0 = (uint32_t)((uint64_t)ntiles * xcd / 8);
   const uint32_t t1 = (uint32_t)((uint64_t)ntiles * (xcd + 1) / 8);
+  float4* __restrict__ part4 = reinterpret_cast<float4*>(part);
   for (uint32_t tile = t0 + (blockIdx.x >> 3); tile < t1; tile += per_xcd) {
     const uint4 d0 = tiles[tile], d1 = tiles[tile + 1];
     const uint32_t s0 = d0.x, e0 = d0.y, nseg = d1.x - s0, count = d1.y - e0;
     __syncthreads();  // the previous tile's readers are done with the LDS arrays
     const bool staged = count <= kPTile;
     if (staged)
-      for (uint32_t i = t; i < count; i += kBlock) {
-        sval[i] = fval[e0 + i];
-        scol[i] = fcol[e0 + i];
+      for (uint32_t i = t; i < count; i += kBlock) {  // streamed once: non-temporal
+        sval[i] = __builtin_nontemporal_load(fval + e0 + i);
+        scol[i] = __builtin_nontemporal_load(fcol + e0 + i);
       }
     for (uint32_t i = t; i <= nseg; i += kBlock) S[i] = fseg[s0 + i] - e0;
     for (uint32_t i = t; i < nseg; i += kBlock) SL[i] = fslot[s0 + i];
     __syncthreads();
-    const float* Xb = X + (size_t)d0.z * W + f;
+    const auto rx = rsrc(X, xbytes);
+    const uint32_t off0 = d0.z * (W * 4u) + q * 16u;
     for (uint32_t sg = grp; sg < nseg; sg += G) {
-      const float d = staged ? pdot<W, kUnroll>(sval, scol, S[sg], S[sg + 1], Xb)
-                             : pdot<W, 4>(fval + e0, fcol + e0, S[sg], S[sg + 1], Xb);
-      part[(size_t)SL[sg] * W + f] = d;
+      const float4 d = staged ? pdot<W, kUnroll>(sval, scol, S[sg], S[sg + 1], rx, off0)
+                              : pdot<W, 4>(fval + e0, fcol + e0, S[sg], S[sg + 1], rx, off0);
+      store_nt(part4 + (size_t)SL[sg] * L + q, d);
     }
   }
 }
@@ -151,30 +194,32 @@ template <int W>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_panel_bwd(
     const uint32_t* __restrict__ bseg, const uint32_t* __restrict__ bslot,
     const uint2* __restrict__ bent, const uint4* __restrict__ tiles, uint32_t ntiles,
-    const float* __restrict__ R, float* __restrict__ part) {
-  constexpr uint32_t G = kBlock / W;
+    const float* __restrict__ R, uint32_t rbytes, float* __restrict__ part) {
+  constexpr uint32_t L = Quad<W>::L, G = Quad<W>::G;
   __shared__ uint2 sent[kPTile];
   __shared__ uint32_t S[kBlock + 1];
   __shared__ uint32_t SL[kBlock];
-  const uint32_t t = threadIdx.x, f = t % W, grp = t / W;
+  const uint32_t t = threadIdx.x, q = t % L, grp = t / L;
   const uint32_t xcd = blockIdx.x & 7u, per_xcd = gridDim.x >> 3;
   const uint32_t t0 = (uint32_t)((uint64_t)ntiles * xcd / 8);
   const uint32_t t1 = (uint32_t)((uint64_t)ntiles * (xcd + 1) / 8);
-  const float* Rf = R + f;
+  const auto rr = rsrc(R, rbytes);
+  const uint32_t off0 = q * 16u;
+  float4* __restrict__ part4 = reinterpret_cast<float4*>(part);
   for (uint32_t tile = t0 + (blockIdx.x >> 3); tile < t1; tile += per_xcd) {
     const uint4 d0 = tiles[tile], d1 = tiles[tile + 1];
     const uint32_t s0 = d0.x, e0 = d0.y, nseg = d1.x - s0, count = d1.y - e0;
     __syncthreads();
     const bool staged = count <= kPTile;
     if (staged)
-      for (uint32_t i = t; i < count; i += kBlock) sent[i] = bent[e0 + i];
+      for (uint32_t i = t; i < count; i += kBlock) sent[i] = load_nt(bent + e0 + i);
     for (uint32_t i = t; i <= nseg; i += kBlock) S[i] = bseg[s0 + i] - e0;
     for (uint32_t i = t; i < nseg; i += kBlock) SL[i] = bslot[s0 + i];
     __syncthreads();
     for (uint32_t sg = grp; sg < nseg; sg += G) {
-      const float d = staged ? rdot<W, kUnroll>(sent, S[sg], S[sg + 1], Rf)
-                             : rdot<W, 4>(bent + e0, S[sg], S[sg + 1], Rf);
-      part[(size_t)SL[sg] * W + f] = d;
+      const float4 d = staged ? rdot<W, kUnroll>(sent, S[sg], S[sg + 1], rr, off0)
+                              : rdot<W, 4>(bent + e0, S[sg], S[sg + 1], rr, off0);
+      store_nt(part4 + (size_t)SL[sg] * L + q, d);
     }
   }
 }
@@ -250,6 +295,8 @@ const PanelFormat& ensure_panels(DevMatrix& m) {
   }
 
   std::unique_ptr<PanelFormat> P(new PanelFormat());
+  P->rows = m.rows;
+  P->cols = m.cols;
   P->npanels = H.npanels;
   P->nchunks = nchunks;
   P->nseg = H.fseg.size() - 1;
@@ -282,11 +329,18 @@ void free_panels(DevMatrix& m) {
   m.panels = nullptr;
 }
 
+// Byte size of a [n][W] fp32 block for a buffer resource (32-bit sizes).
+uint32_t block_bytes(uint64_t n, int W) {
+  const uint64_t b = 4ull * n * (uint64_t)W;
+  if (b > 0xFFFFFFFFull) raise(MLEM_EINVAL, "right-hand-side block larger than 4 GiB");
+  return (uint32_t)b;
+}
+
 template <int W>
 void launch_panel_fwd(const PanelFormat& P, const float* X, float* part, uint32_t grid,
                       hipStream_t s) {
   hipLaunchKernelGGL((k_panel_fwd<W>), dim3(grid), dim3(kBlock), 0, s, P.fseg, P.fslot, P.fval,
-                     P.fcol, P.ftiles, P.nftiles, X, part);
+                     P.fcol, P.ftiles, P.nftiles, X, block_bytes(P.cols, W), part);
   check_launch("tiled forward launch");
 }
 
@@ -294,7 +348,7 @@ template <int W>
 void launch_panel_bwd(const PanelFormat& P, const float* R, float* part, uint32_t grid,
                       hipStream_t s) {
   hipLaunchKernelGGL((k_panel_bwd<W>), dim3(grid), dim3(kBlock), 0, s, P.bseg, P.bslot, P.bent,
-                     P.btiles, P.nbtiles, R, part);
+                     P.btiles, P.nbtiles, R, block_bytes(P.rows, W), part);
   check_launch("tiled backward launch");
 }
 
