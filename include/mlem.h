@@ -256,6 +256,25 @@ MLEM_API int mlem_reconstruct(mlem_matrix* m, const int32_t* y, uint32_t rows,
                      float* x_inout, uint32_t cols, int iterations,
                      const mlem_options* opts);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the iterations of mlem_reconstruct on device-resident
+ * vectors, allocating nothing. y (rows int32 counts) and x (cols fp32 image:
+ * x0 in, x_K out) are DEVICE pointers in file row / voxel order; mode is
+ * MLEM_MODE_FAST or MLEM_MODE_EXACT; loglik and fwd_total (host,
+ * [iterations], may be NULL) receive the FP64 statistics of every iteration.
+ * The work runs on the matrix's stream (mlem_stream): make y and x ready
+ * before the call. With both statistics arrays NULL the call only enqueues
+ * (no synchronization; it can be captured into a HIP graph), otherwise it
+ * returns when done. Its scratch (device-order copies of y and x, the ratio
+ * vector, FP64 statistics partials) is one static 64 MiB device array per
+ * GPU, shared by every call on that GPU: calls that overlap on one GPU race.
+ * Matrices that need more scratch (4 * (2 rows + cols) bytes plus the
+ * statistics) get MLEM_EINVAL; streamed matrices are not supported. */
+MLEM_API int mlem_reconstruct_noalloc(mlem_matrix* m, const int32_t* y, uint32_t rows,
+                                      float* x, uint32_t cols, int iterations, int mode,
+                                      double* loglik, double* fwd_total);
+
 /* ---- device-resident reconstruction state --------------------------------
  * The same iteration as mlem_reconstruct, split so that the sinogram and the
  * image stay in HBM between calls (used by the benchmark and the batched

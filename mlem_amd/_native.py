@@ -93,6 +93,7 @@ SIGNATURES = {
     "mlem_update": (C.c_int, [_P, _P, _P, _P, _u32]),
     "mlem_options_default": (None, [C.POINTER(Options)]),
     "mlem_reconstruct": (C.c_int, [_P, _P, _u32, _P, _u32, C.c_int, C.POINTER(Options)]),
+    "mlem_reconstruct_noalloc": (C.c_int, [_P, _P, _u32, _P, _u32, C.c_int, C.c_int, _P, _P]),
     "mlem_recon_create": (C.c_int, [_P, _P, _u32, _P, _u32, C.POINTER(Options), C.POINTER(_P)]),
     "mlem_recon_destroy": (None, [_P]),
     "mlem_recon_run": (C.c_int, [_P, C.c_int]),

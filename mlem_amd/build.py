@@ -58,7 +58,7 @@ def _hip_object(src: Path) -> Path:
 def build_libmlem() -> Path:
     srcs = [CSRC / "kernels.hip", CSRC / "layout.hip", CSRC / "batch.hip", CSRC / "unroll.hip",
             CSRC / "api.cpp", CSRC / "csr4.cpp", CSRC / "symmetry.cpp", CSRC / "stream.cpp",
-            CSRC / "panel.hip", CSRC / "panels.cpp"]
+            CSRC / "panel.hip", CSRC / "panels.cpp", CSRC / "noalloc.cpp"]
     with ThreadPoolExecutor(max_workers=len(srcs)) as ex:
         objs = list(ex.map(_hip_object, srcs))
     out = LIB / "libmlem.so"
