@@ -438,15 +438,27 @@ def main() -> int:
         name = "k_scatter + k_update (atomic backprojection)"
 
     # HBM bytes per launch of the dominant kernel: the L2-to-memory read and
-    # write requests of the committed rocprofv3 counter passes for this
-    # workload and launch configuration, times their request sizes.
+    # write requests of the committed rocprofv3 counter passes over this
+    # bench's workload, times their request sizes (mlem_amd/counters.py).
     traffic = None
     ptag = "" if args.precision == "fp32" else f"_{args.precision}"
-    tfile = ROOT / "profiles" / f"traffic_{args.workload}_{args.mode}{ptag}.json"
-    if tfile.exists():
-        t = json.loads(tfile.read_text())
-        if t.get("launch") == json.loads(json.dumps(launch)):
-            traffic = t.get(dom)
+    tfile = ROOT / "profiles" / f"traffic_{args.workload}{ptag}.json"
+    fam, ept = launch[dom]
+    if tfile.exists() and fam in ("dict", "dict512") and args.mode == "fast":
+        from mlem_amd.counters import kernel_traffic
+        vk = {"fp32": 0, "fp16": 1, "bf16": 2}[args.precision]
+        nb = 512 if fam == "dict512" else 256
+        # the timed launches: FAST, fused ratio (forward, no statistics) or
+        # fused update (backward); else the autotuner's launches of the same
+        # tile configuration (the same elements streamed, no update)
+        summary = json.loads(tfile.read_text())
+        tmpls = [f"<{nb}, {ept}, {vk}, false, true, false>("] if dom == "fwd" else \
+            [f"<{nb}, {ept}, {vk}, false, true>(", f"<{nb}, {ept}, {vk}, false, false>("]
+        for tmpl in tmpls:
+            k = kernel_traffic(summary, f"k_{dom}_dict{tmpl}")
+            if k and "bytes" in k:
+                traffic = k["bytes"]
+                break
 
     cpu = None
     if args.cpu_iters > 0:

@@ -16,8 +16,9 @@ TCC counters):
     rocprofv3 --pmc TCC_EA0_WRREQ_sum TCC_EA0_WRREQ_64B_sum -d <dirW> -o run \\
               --output-format csv -- <cmd>
 
-then `python3 -m mlem_amd.counters <dirR> <dirW>` prints, per kernel name, the
-dispatch count and the mean read / write bytes per dispatch as JSON.
+then `python3 -m mlem_amd.counters <dirR> <dirW> [--command TEXT]` prints, per
+kernel name, the dispatch count and the mean read / write bytes per dispatch
+as JSON (with TEXT, the command that produced the passes, under "command").
 """
 from __future__ import annotations
 
@@ -71,11 +72,30 @@ def summarize(read_dir: Path, write_dir: Path) -> dict:
     return result
 
 
+def kernel_traffic(summary: dict, prefix: str):
+    """The entry of the first kernel whose name contains `prefix` (or None)."""
+    for name in sorted(summary.get("kernels", summary)):
+        if prefix in name:
+            return summary.get("kernels", summary)[name]
+    return None
+
+
 def main(argv) -> int:
-    if len(argv) != 3:
+    args = list(argv[1:])
+    command = None
+    if "--command" in args:
+        i = args.index("--command")
+        command = args[i + 1]
+        del args[i:i + 2]
+    if len(args) != 2:
         print(__doc__, file=sys.stderr)
         return 2
-    json.dump(summarize(Path(argv[1]), Path(argv[2])), sys.stdout, indent=1, sort_keys=True)
+    out = {"kernels": summarize(Path(args[0]), Path(args[1])),
+           "formula": "read = 32*RDREQ_32B + 64*RDREQ_64B + 128*RDREQ_128B; "
+                      "write = 64*WRREQ_64B + 32*(WRREQ - WRREQ_64B) (TCC_EA0_*_sum, per dispatch)"}
+    if command:
+        out["command"] = command
+    json.dump(out, sys.stdout, indent=1, sort_keys=True)
     print()
     return 0
 

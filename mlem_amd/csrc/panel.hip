@@ -52,17 +52,15 @@ void dfree(T*& p) {
   p = nullptr;
 }
 
+static_assert(kMaxSegment <= kPTile, "every segment fits a staged tile");
+
 // Greedy tiles over segments [s, e) with element offsets off[]: at most kBlock
-// segments and kPTile elements per tile; a longer segment forms its own tile.
+// segments and kPTile elements per tile (no segment is longer than that).
 void add_tiles(const std::vector<uint32_t>& off, uint32_t s, uint32_t e, uint32_t base,
                std::vector<uint4>* tiles) {
   while (s < e) {
-    uint32_t t = s;
-    if (off[s + 1] - off[s] > kPTile) {
-      t = s + 1;
-    } else {
-      while (t < e && t - s < (uint32_t)kBlock && off[t + 1] - off[s] <= kPTile) ++t;
-    }
+    uint32_t t = s + 1;
+    while (t < e && t - s < (uint32_t)kBlock && off[t + 1] - off[s] <= kPTile) ++t;
     tiles->push_back(make_uint4(s, off[s], base, 0));
     s = t;
   }
@@ -70,13 +68,31 @@ void add_tiles(const std::vector<uint32_t>& off, uint32_t s, uint32_t e, uint32_
 
 // A lane carries four right-hand sides (one 16-byte gather per element): the
 // address unit handles a fixed number of lanes per clock, so 16-byte lanes
-// move four times the bytes of 4-byte ones. W/4 lanes share a segment.
+// move four times the bytes of 4-byte ones. Every segment gets 8 lanes
+// whatever W is: W/4 lanes for the quads times E = 32/W lanes that take
+// every E-th element, their sums then added across the E lanes by a fixed
+// butterfly, so narrow blocks keep as many gathers in flight as wide ones.
 template <int W>
 struct Quad {
-  static_assert(W % 4 == 0, "right-hand sides come in quads");
-  static constexpr uint32_t L = W / 4;        // lanes per segment
-  static constexpr uint32_t G = kBlock / L;   // segments in flight per workgroup
+  static_assert(W % 4 == 0 && W <= 32, "right-hand sides come in quads, at most 32");
+  static constexpr uint32_t L = W / 4;        // lanes per element (quads)
+  static constexpr uint32_t E = 8 / L;        // lanes per segment and quad
+  static constexpr uint32_t G = kBlock / 8;   // segments in flight per workgroup
 };
+
+// Sum over the E lanes (lane distance L) of a segment and quad; every one of
+// them ends with the same value (each step adds a pair in either order).
+template <int W>
+__device__ __forceinline__ float4 across_e(float4 v) {
+#pragma unroll
+  for (uint32_t m = Quad<W>::L; m < 8; m *= 2) {
+    v.x += __shfl_xor(v.x, (int)m);
+    v.y += __shfl_xor(v.y, (int)m);
+    v.z += __shfl_xor(v.z, (int)m);
+    v.w += __shfl_xor(v.w, (int)m);
+  }
+  return v;
+}
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
@@ -111,40 +127,44 @@ __device__ __forceinline__ void fma4(float a, const float4& g, float4& acc) {
   acc.w = fmaf(a, g.w, acc.w);
 }
 
-// Dot products of one segment (elements [b, e) of vals / cols, LDS when the
-// tile is staged, else global) with the lane's four vectors: U gathers
-// issued together, then fused multiply-adds in element order. off0 is the
-// byte offset in X of the lane's quad of the panel's first column; each
-// further column is W*4 bytes on.
+// This lane's share of one staged segment's dot products (elements b + ei,
+// b + ei + E, ... of [b, e)) with its four vectors: U gathers issued
+// together, then fused multiply-adds in element order. off0 is the byte
+// offset in X of the lane's quad of the panel's first column; each further
+// column is W*4 bytes on.
 template <int W, int U>
 __device__ __forceinline__ float4 pdot(const float* vals, const uint16_t* cols, uint32_t b,
-                                       uint32_t e, __amdgpu_buffer_rsrc_t rx, uint32_t off0) {
+                                       uint32_t e, uint32_t ei, __amdgpu_buffer_rsrc_t rx,
+                                       uint32_t off0) {
+  constexpr uint32_t E = Quad<W>::E;
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (uint32_t k = b; k < e; k += U) {
+  for (uint32_t k = b + ei; k < e; k += U * E) {
     float4 g[U];
 #pragma unroll
-    for (int j = 0; j < U; ++j) g[j] = gather4(rx, off0 + (uint32_t)cols[min(k + j, e - 1)] * (W * 4u));
+    for (int j = 0; j < U; ++j)
+      g[j] = gather4(rx, off0 + (uint32_t)cols[min(k + j * E, e - 1)] * (W * 4u));
     // the values are read once the gathers are in flight (fewer live registers)
 #pragma unroll
     for (int j = 0; j < U; ++j)
-      if (k + j < e) fma4(vals[min(k + j, e - 1)], g[j], acc);
+      if (k + j * E < e) fma4(vals[k + j * E], g[j], acc);
   }
-  return acc;
+  return across_e<W>(acc);
 }
 
 template <int W, int U>
-__device__ __forceinline__ float4 rdot(const uint2* ent, uint32_t b, uint32_t e,
+__device__ __forceinline__ float4 rdot(const uint2* ent, uint32_t b, uint32_t e, uint32_t ei,
                                        __amdgpu_buffer_rsrc_t rr, uint32_t off0) {
+  constexpr uint32_t E = Quad<W>::E;
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (uint32_t k = b; k < e; k += U) {
+  for (uint32_t k = b + ei; k < e; k += U * E) {
     float4 g[U];
 #pragma unroll
-    for (int j = 0; j < U; ++j) g[j] = gather4(rr, off0 + ent[min(k + j, e - 1)].x * (W * 4u));
+    for (int j = 0; j < U; ++j) g[j] = gather4(rr, off0 + ent[min(k + j * E, e - 1)].x * (W * 4u));
 #pragma unroll
     for (int j = 0; j < U; ++j)
-      if (k + j < e) fma4(__uint_as_float(ent[min(k + j, e - 1)].y), g[j], acc);
+      if (k + j * E < e) fma4(__uint_as_float(ent[k + j * E].y), g[j], acc);
   }
-  return acc;
+  return across_e<W>(acc);
 }
 
 // Forward tiles: part[fslot[s] * W + f] = sum over segment s of a * X[.].
@@ -156,12 +176,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     const float* __restrict__ fval, const uint16_t* __restrict__ fcol,
     const uint4* __restrict__ tiles, uint32_t ntiles, const float* __restrict__ X,
     uint32_t xbytes, float* __restrict__ part) {
-  constexpr uint32_t L = Quad<W>::L, G = Quad<W>::G;
+  constexpr uint32_t L = Quad<W>::L, E = Quad<W>::E, G = Quad<W>::G;
   __shared__ float sval[kPTile];
   __shared__ uint16_t scol[kPTile];
   __shared__ uint32_t S[kBlock + 1];
   __shared__ uint32_t SL[kBlock];
-  const uint32_t t = threadIdx.x, q = t % L, grp = t / L;
+  const uint32_t t = threadIdx.x, q = t % L, ei = (t / L) % E, grp = t / 8;
   const uint32_t xcd = blockIdx.x & 7u, per_xcd = gridDim.x >> 3;
   const uint32_t t0 = (uint32_t)((uint64_t)ntiles * xcd / 8);
   const uint32_t t1 = (uint32_t)((uint64_t)ntiles * (xcd + 1) / 8);
@@ -170,21 +190,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     const uint4 d0 = tiles[tile], d1 = tiles[tile + 1];
     const uint32_t s0 = d0.x, e0 = d0.y, nseg = d1.x - s0, count = d1.y - e0;
     __syncthreads();  // the previous tile's readers are done with the LDS arrays
-    const bool staged = count <= kPTile;
-    if (staged)
-      for (uint32_t i = t; i < count; i += kBlock) {  // streamed once: non-temporal
-        sval[i] = __builtin_nontemporal_load(fval + e0 + i);
-        scol[i] = __builtin_nontemporal_load(fcol + e0 + i);
-      }
+    for (uint32_t i = t; i < count; i += kBlock) {  // streamed once: non-temporal
+      sval[i] = __builtin_nontemporal_load(fval + e0 + i);
+      scol[i] = __builtin_nontemporal_load(fcol + e0 + i);
+    }
     for (uint32_t i = t; i <= nseg; i += kBlock) S[i] = fseg[s0 + i] - e0;
     for (uint32_t i = t; i < nseg; i += kBlock) SL[i] = fslot[s0 + i];
     __syncthreads();
     const auto rx = rsrc(X, xbytes);
     const uint32_t off0 = d0.z * (W * 4u) + q * 16u;
     for (uint32_t sg = grp; sg < nseg; sg += G) {
-      const float4 d = staged ? pdot<W, kUnroll>(sval, scol, S[sg], S[sg + 1], rx, off0)
-                              : pdot<W, 4>(fval + e0, fcol + e0, S[sg], S[sg + 1], rx, off0);
-      store_nt(part4 + (size_t)SL[sg] * L + q, d);
+      const float4 d = pdot<W, kUnroll>(sval, scol, S[sg], S[sg + 1], ei, rx, off0);
+      if (ei == 0) store_nt(part4 + (size_t)SL[sg] * L + q, d);
     }
   }
 }
@@ -195,11 +212,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     const uint32_t* __restrict__ bseg, const uint32_t* __restrict__ bslot,
     const uint2* __restrict__ bent, const uint4* __restrict__ tiles, uint32_t ntiles,
     const float* __restrict__ R, uint32_t rbytes, float* __restrict__ part) {
-  constexpr uint32_t L = Quad<W>::L, G = Quad<W>::G;
+  constexpr uint32_t L = Quad<W>::L, E = Quad<W>::E, G = Quad<W>::G;
   __shared__ uint2 sent[kPTile];
   __shared__ uint32_t S[kBlock + 1];
   __shared__ uint32_t SL[kBlock];
-  const uint32_t t = threadIdx.x, q = t % L, grp = t / L;
+  const uint32_t t = threadIdx.x, q = t % L, ei = (t / L) % E, grp = t / 8;
   const uint32_t xcd = blockIdx.x & 7u, per_xcd = gridDim.x >> 3;
   const uint32_t t0 = (uint32_t)((uint64_t)ntiles * xcd / 8);
   const uint32_t t1 = (uint32_t)((uint64_t)ntiles * (xcd + 1) / 8);
@@ -210,16 +227,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     const uint4 d0 = tiles[tile], d1 = tiles[tile + 1];
     const uint32_t s0 = d0.x, e0 = d0.y, nseg = d1.x - s0, count = d1.y - e0;
     __syncthreads();
-    const bool staged = count <= kPTile;
-    if (staged)
-      for (uint32_t i = t; i < count; i += kBlock) sent[i] = load_nt(bent + e0 + i);
+    for (uint32_t i = t; i < count; i += kBlock) sent[i] = load_nt(bent + e0 + i);
     for (uint32_t i = t; i <= nseg; i += kBlock) S[i] = bseg[s0 + i] - e0;
     for (uint32_t i = t; i < nseg; i += kBlock) SL[i] = bslot[s0 + i];
     __syncthreads();
     for (uint32_t sg = grp; sg < nseg; sg += G) {
-      const float4 d = staged ? rdot<W, kUnroll>(sent, S[sg], S[sg + 1], rr, off0)
-                              : rdot<W, 4>(bent + e0, S[sg], S[sg + 1], rr, off0);
-      store_nt(part4 + (size_t)SL[sg] * L + q, d);
+      const float4 d = rdot<W, kUnroll>(sent, S[sg], S[sg + 1], ei, rr, off0);
+      if (ei == 0) store_nt(part4 + (size_t)SL[sg] * L + q, d);
     }
   }
 }
@@ -280,7 +294,6 @@ const PanelFormat& ensure_panels(DevMatrix& m) {
     raise(MLEM_EINVAL, std::string("matrix tiling: ") + e.what());
   }
   std::vector<uint32_t>().swap(csr2);
-
   std::vector<uint4> ft, bt;
   for (uint32_t p = 0; p < H.npanels; ++p) add_tiles(H.fseg, H.pseg[p], H.pseg[p + 1], H.pstart[p], &ft);
   ft.push_back(make_uint4((uint32_t)(H.fseg.size() - 1), (uint32_t)m.nnz, 0, 0));

@@ -122,18 +122,27 @@ void build_host_panels(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_
   std::vector<uint32_t> nsegrow(rows);
   std::vector<std::vector<uint64_t>> cseg((size_t)T, std::vector<uint64_t>(P, 0));
   std::vector<std::vector<uint64_t>> cel((size_t)T, std::vector<uint64_t>(P, 0));
+  const uint32_t SP = kMaxSegment;
   parallel_for(T, [&](int t) {
-    std::vector<uint32_t> stamp(P, 0xFFFFFFFFu);
+    std::vector<uint32_t> stamp(P, 0xFFFFFFFFu), cnt(P, 0);
+    std::vector<uint32_t> plist;
     for (uint32_t r = rr[(size_t)t]; r < rr[(size_t)t + 1]; ++r) {
-      uint32_t nd = 0;
+      plist.clear();
       for (uint32_t k = rowptr[r]; k < rowptr[r + 1]; ++k) {
         const uint32_t p = pid[csr2[2ull * k]];
         cel[(size_t)t][p]++;
         if (stamp[p] != r) {
           stamp[p] = r;
-          ++nd;
-          cseg[(size_t)t][p]++;
+          cnt[p] = 0;
+          plist.push_back(p);
         }
+        cnt[p]++;
+      }
+      uint32_t nd = 0;
+      for (uint32_t p : plist) {  // a (row, panel) piece longer than SP is split
+        const uint32_t pieces = (cnt[p] + SP - 1) / SP;
+        nd += pieces;
+        cseg[(size_t)t][p] += pieces;
       }
       nsegrow[r] = nd;
     }
@@ -184,13 +193,16 @@ void build_host_panels(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_
         cnt[p]++;
       }
       std::sort(plist.begin(), plist.end());
+      uint32_t rank = 0;
       for (size_t j = 0; j < plist.size(); ++j) {
         const uint32_t p = plist[j];
-        const uint64_t s = scur[(size_t)t][p]++;
-        H.fseg[s] = (uint32_t)ecur[(size_t)t][p];
-        H.fslot[s] = H.rowslot[r] + (uint32_t)j;
-        segrow[s] = r;
         pos[p] = ecur[(size_t)t][p];
+        for (uint32_t done = 0; done < cnt[p]; done += SP) {
+          const uint64_t s = scur[(size_t)t][p]++;
+          H.fseg[s] = (uint32_t)(ecur[(size_t)t][p] + done);
+          H.fslot[s] = H.rowslot[r] + rank++;
+          segrow[s] = r;
+        }
         ecur[(size_t)t][p] += cnt[p];
       }
       for (uint32_t k = rowptr[r]; k < rowptr[r + 1]; ++k) {
@@ -232,8 +244,8 @@ void build_host_panels(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_
       }
       uint64_t ng = 0;
       for (uint32_t v = 0; v < pv; ++v) {
-        uint32_t n = 0;
-        for (uint32_t k = 0; k < nk; ++k) n += g[(size_t)k * pv + v] ? 1u : 0u;
+        uint32_t n = 0;  // groups of the voxel: non-empty chunks, long ones split
+        for (uint32_t k = 0; k < nk; ++k) n += (g[(size_t)k * pv + v] + SP - 1) / SP;
         nchunk_pc[pstart[p] + v] = n;
         ng += n;
       }
@@ -267,11 +279,13 @@ void build_host_panels(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_
         for (uint32_t v = 0; v < pv; ++v) {
           const uint32_t n = g[(size_t)k * pv + v];
           if (!n) continue;
-          H.bseg[gi] = (uint32_t)e;
-          H.bslot[gi] = H.colslot[pstart[p] + v] + rank[v]++;
           gpos[(size_t)k * pv + v] = e;
+          for (uint32_t done = 0; done < n; done += SP) {
+            H.bseg[gi] = (uint32_t)(e + done);
+            H.bslot[gi] = H.colslot[pstart[p] + v] + rank[v]++;
+            ++gi;
+          }
           e += n;
-          ++gi;
         }
       }
       for (uint32_t s = H.pseg[p]; s < H.pseg[p + 1]; ++s) {

@@ -32,6 +32,10 @@
 
 namespace mlem {
 
+// Longest segment (forward) or group (backward): longer pieces are split into
+// several, each with its own slot, so that every segment fits a staged tile.
+constexpr uint32_t kMaxSegment = 512;
+
 struct HostPanels {
   uint32_t rows = 0, cols = 0;
   uint64_t nnz = 0;
