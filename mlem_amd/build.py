@@ -29,11 +29,12 @@ ARCH = "gfx950"
 
 # Flags shared by every HIP translation unit:
 #  -ffp-contract=off       multiply/add fuse only where a kernel asks for fmaf
-#  -fgpu-flush-denormals-to-zero   fp32 flush-to-zero (the reference sets FTZ)
+#  -fdenormal-fp-math-f32=preserve-sign,ieee   fp32 subnormal results flushed,
+#                          subnormal inputs kept (the reference sets FTZ, not DAZ)
 #  -packed-fp32-ops off    no packed fp32 instructions are generated
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden",
-    "-ffp-contract=off", "-fgpu-flush-denormals-to-zero",
+    "-ffp-contract=off", "-Xclang", "-fdenormal-fp-math-f32=preserve-sign,ieee",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result", f"-I{INCLUDE}", f"-I{CSRC}",
 ]

@@ -121,7 +121,7 @@ struct FwdRatio {
   __device__ void operator()(uint32_t row, uint32_t f, float F) {
     const size_t i = (size_t)row * W + f;
     const int32_t yi = Y[i];
-    R[i] = (F != 0.f) ? (float)yi / F : 0.f;
+    R[i] = (F != 0.f) ? div_rn((float)yi, F) : 0.f;
     Fp[i] = F;
   }
 };
@@ -137,7 +137,7 @@ struct BwdUpdate {
     if (!((mask >> f) & 1u)) return;
     const float n = norm[col];
     const size_t i = (size_t)col * W + f;
-    X[i] = X[i] * ((n != 0.f) ? u / n : u);
+    X[i] = X[i] * ((n != 0.f) ? div_rn(u, n) : u);
   }
 };
 
@@ -227,7 +227,7 @@ __device__ __forceinline__ void add4(float4& a, const float4& b) {
   a.w += b.w;
 }
 
-__device__ __forceinline__ float ratio1(int32_t y, float f) { return (f != 0.f) ? (float)y / f : 0.f; }
+__device__ __forceinline__ float ratio1(int32_t y, float f) { return (f != 0.f) ? div_rn((float)y, f) : 0.f; }
 
 __device__ __forceinline__ void stat1(int32_t y, float f, double& ll, double& tot) {
   tot += (double)f;
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
   batch_stats_partial4<W>(ll, tot, partials);
 }
 
-__device__ __forceinline__ float update1(float x, float u, float n) { return x * ((n != 0.f) ? u / n : u); }
+__device__ __forceinline__ float update1(float x, float u, float n) { return x * ((n != 0.f) ? div_rn(u, n) : u); }
 
 // Epilogue of the tiled backprojection (FAST mode): u = the voxel's group
 // slots added in chunk order, then x *= (n != 0) ? u/n : u for live lanes.

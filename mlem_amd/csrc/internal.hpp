@@ -20,6 +20,26 @@ struct ApiError {
   std::string msg;
 };
 
+// Every kernel runs in the fp32 mode of the reference driver, which sets
+// flush-to-zero but not denormals-are-zero (mlem_seq/seqcsr4mlem.cpp:194):
+// subnormal inputs are used as they are, subnormal results become zero. The
+// code is built with -fdenormal-fp-math-f32=preserve-sign,ieee (MODE register
+// FP32 denormal field 1, "flush outputs").
+//
+// fp32 division goes through div_rn: the compiler's own fp32 division would
+// switch to full denormals for its fused steps and then write field 0
+// (inputs flushed as well) for the rest of the kernel. div_rn divides in
+// fp64 instead (fp64 keeps denormals, no mode switch): the quotient of two
+// fp32 values rounded to fp64 and then to fp32 is the correctly rounded fp32
+// quotient, since 53 >= 2*24 + 2. As x86 under FTZ, a quotient is flushed to
+// zero when it is tiny after rounding, i.e. when its rounding to 24 bits with
+// unbounded exponent is below 2^-126, which is |a/b| < 2^-126 - 2^-151.
+__device__ __forceinline__ float div_rn(float a, float b) {
+  const double q = (double)a / (double)b;
+  constexpr double kTiny = 0x1.ffffffp-127;  // 2^-126 - 2^-151
+  return (float)(__builtin_fabs(q) < kTiny ? __builtin_copysign(0.0, q) : q);
+}
+
 [[noreturn]] void raise(int code, const std::string& msg);
 void hip_check(hipError_t e, const char* what);
 #define MLEM_HIP(call) ::mlem::hip_check((call), #call)

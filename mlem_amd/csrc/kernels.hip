@@ -211,7 +211,7 @@ struct FwdEpilogue {
   }
   __device__ void finish(uint32_t row, float f) {
     if constexpr (RATIO) {
-      out[row] = (f != 0.f) ? (float)yi / f : 0.f;
+      out[row] = (f != 0.f) ? div_rn((float)yi, f) : 0.f;
       if constexpr (STATS) {
         const double fd = (double)f * (double)s;
         tot += fd;
@@ -237,7 +237,7 @@ struct BwdEpilogue {
   }
   __device__ void finish(uint32_t col, float u) {
     if constexpr (UPDATE)
-      out[col] = x * ((n != 0.f) ? u / n : u);
+      out[col] = x * ((n != 0.f) ? div_rn(u, n) : u);
     else
       out[col] = u;
   }
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd(const uint32_t* __restrict__ col
     if (active && lane == 0) {
       if constexpr (OUT == kBwdUpdate) {
         const float n = norm[col];
-        out[col] = out[col] * ((n != 0.f) ? acc / n : acc);
+        out[col] = out[col] * ((n != 0.f) ? div_rn(acc, n) : acc);
       } else if constexpr (OUT == kBwdAdd && !EXACT) {
         out[col] = out[col] + acc;
       } else {
@@ -935,7 +935,7 @@ __global__ __launch_bounds__(kBlock) void k_update(uint32_t cols, const float* _
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= cols) return;
   const float n = norm[j], uj = u[j];
-  x[j] = x[j] * ((n != 0.f) ? uj / n : uj);
+  x[j] = x[j] * ((n != 0.f) ? div_rn(uj, n) : uj);
   if (zero_u) zero_u[j] = 0.f;
 }
 
@@ -945,7 +945,7 @@ __global__ __launch_bounds__(kBlock) void k_correl(uint32_t rows, const float* _
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= rows) return;
   const float fi = f[i];
-  r[i] = (fi != 0.f) ? (float)y[i] / fi : 0.f;
+  r[i] = (fi != 0.f) ? div_rn((float)y[i], fi) : 0.f;
 }
 
 // Column sums from the CSC: one lane per column, ascending rows, fp32 — the
@@ -1002,7 +1002,7 @@ __global__ __launch_bounds__(kBlock) void k_quantize_rows(const uint32_t* __rest
   if (!live) return;
   const float s = (mx > 0.f) ? mx : 1.f;
   for (uint32_t k = b + lane; k < e; k += G)
-    csr[k].y = __float_as_uint(quantize_value<VK>(__uint_as_float(csr[k].y) / s));
+    csr[k].y = __float_as_uint(quantize_value<VK>(div_rn(__uint_as_float(csr[k].y), s)));
   if (lane == 0) rscale[row] = s;
 }
 

@@ -99,12 +99,12 @@ __global__ __launch_bounds__(kBlock) void k_row_keys(const uint32_t* __restrict_
   float phi = atan2f(y1 - y0, x1 - x0);
   if (phi < 0.f) phi += pi;
   if (phi >= pi) phi -= pi;
-  const float mx = sx / n - 0.5f * nx, my = sy / n - 0.5f * ny, mz = sz / n;
+  const float mx = div_rn(sx, n) - 0.5f * nx, my = div_rn(sy, n) - 0.5f * ny, mz = div_rn(sz, n);
   const float s = -sinf(phi) * mx + cosf(phi) * my;
   const float smax = 0.5f * sqrtf((float)nx * nx + (float)ny * ny) + 1.f;
-  const uint32_t pb = min((uint32_t)(phi / pi * 64.f), 63u);
-  const uint32_t sb = (uint32_t)min(max((s + smax) / (2.f * smax) * 256.f, 0.f), 255.f);
-  const uint32_t zb = (uint32_t)min(max(mz / (float)nz * 64.f, 0.f), 63.f);
+  const uint32_t pb = min((uint32_t)(div_rn(phi, pi) * 64.f), 63u);
+  const uint32_t sb = (uint32_t)min(max(div_rn(s + smax, 2.f * smax) * 256.f, 0.f), 255.f);
+  const uint32_t zb = (uint32_t)min(max(div_rn(mz, (float)nz) * 64.f, 0.f), 63.f);
   keys[row] = (pb << 14) | (sb << 6) | zb;
 }
 

@@ -57,7 +57,7 @@ struct StepFwd {
   __device__ void operator()(uint32_t row, uint32_t l, float f) {
     const size_t i = (size_t)row * W + l;
     F[i] = f;
-    R[i] = (f != 0.f) ? Y[i] / f : 0.f;
+    R[i] = (f != 0.f) ? div_rn(Y[i], f) : 0.f;
   }
 };
 
@@ -72,7 +72,7 @@ struct StepBwd {
     const size_t i = (size_t)col * W + l;
     const float n = norm[col];
     if (U) U[i] = u;
-    Xn[i] = X[i] * ((n != 0.f) ? u / n : u);
+    Xn[i] = X[i] * ((n != 0.f) ? div_rn(u, n) : u);
   }
 };
 
@@ -87,8 +87,8 @@ struct AdjFwd {
     const size_t i = (size_t)row * W + l;
     const float f = F[i];
     if (f != 0.f) {
-      GY[i] = GY[i] + gr / f;
-      GF[i] = -(gr * R[i]) / f;
+      GY[i] = GY[i] + div_rn(gr, f);
+      GF[i] = div_rn(-(gr * R[i]), f);
     } else {
       GF[i] = 0.f;
     }
@@ -125,8 +125,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_update(float* __restrict__ G,
   const float nj = norm[i / (uint64_t)W];
   const float g = G[i];
   if (nj != 0.f) {
-    GU[i] = g * (X[i] / nj);
-    G[i] = g * (U[i] / nj);
+    GU[i] = g * div_rn(X[i], nj);
+    G[i] = g * div_rn(U[i], nj);
   } else {
     GU[i] = g * X[i];
     G[i] = g * U[i];

@@ -263,6 +263,59 @@ def test_edge_cases_empty_rows_and_columns():
     A.close()
 
 
+def _matrix(rows, cols):
+    ends = np.cumsum([len(r) for r in rows]).astype(np.uint64)
+    el = np.array([e for r in rows for e in r], dtype=M.ELEMENT_DTYPE)
+    return ends, el, M.Csr4Matrix.from_arrays(len(rows), cols, ends, el)
+
+
+def test_exact_keeps_subnormal_inputs_and_flushes_subnormal_results():
+    # The reference sets FTZ without DAZ (mlem_seq/seqcsr4mlem.cpp:194): a
+    # subnormal matrix value, image value or forward projection is used as it
+    # is; a subnormal product, sum or quotient becomes zero.
+    ends, el, A = _matrix([[(0, 1e10)], [(0, 1e-40)], [(2, 1e-20)], [(1, 1e-39), (0, 3e-39)]], 3)
+    x = np.float32([1e-40, 1e10, 1e-20])
+    f = M.calcFwProj(A, x, "exact")
+    assert np.array_equal(f, orc.fwproj(ends, el, x))
+    assert f[0] == np.float32(1e10) * np.float32(1e-40) > 0      # subnormal image value kept
+    assert f[1] == 0 and f[2] == 0                                # subnormal products flushed
+    assert f[3] == np.float32(1e-39) * np.float32(1e10) > 0      # subnormal matrix value kept
+    fr = np.float32([1e-40, 3e38, 2.0, 0.0])                      # subnormal f: 1/f overflows
+    yr = np.int32([1, 1, 3, 4])
+    r = M.calcCorrel(A, fr, yr)
+    assert np.array_equal(r, orc.correl(fr, yr))
+    assert np.isinf(r[0]) and r[1] == 0                           # tiny quotient flushed
+    rb = np.float32([1e-40, 1e10, 5e37, 1.0])
+    u = M.calcBkProj(A, rb, "exact")
+    assert np.array_equal(u, orc.bkproj(ends, el, rb, 3))
+    assert u[0] > 0 and u[1] == 0                                 # 1e-39 * 1.0 flushed
+    norm = np.float32([1.0, 4.0, 2.0])
+    x = np.float32([1e-40, 1e-40, 1.0])                           # subnormal image, after u/n
+    for uu in (np.float32([1e10, 3.0, 1.0]), np.float32([1e-30, 1e-20, 1e-37])):
+        xg = M.calcUpdate(A, uu, norm, x.copy())
+        assert np.array_equal(xg, orc.update(uu, norm, x))
+    assert M.calcUpdate(A, np.float32([1e10, 3.0, 1.0]), norm, x.copy())[0] > 0
+    A.close()
+    # column sums that stay subnormal are flushed (norm 0, the update then
+    # multiplies by u); rows made only of subnormal values still project
+    rows = [[(0, 1.0), (1, 0.5)], [(0, 1e-39), (2, 2e-39)], [(1, 0.25), (2, 1e-40), (3, 1.0)],
+            [(3, 2.0)]]
+    ends, el, A = _matrix(rows, 4)
+    norm = M.calcColumnSums(A)
+    assert np.array_equal(norm, orc.column_sums(ends, el, 4)) and norm[2] == 0
+    y = np.int32([1000000, 3, 500000, 7])
+    x = M.mlem(A, y, None, 6, mode="exact")
+    xo = orc.mlem(ends, el, 4, y, 6)[0]
+    assert np.array_equal(x, xo)
+    # without the subnormal inputs (denormals-are-zero) the result differs
+    el_daz = el.copy()
+    for k in range(len(el_daz)):
+        if 0 < abs(float(el_daz[k]["value"])) < 1.17549435e-38:
+            el_daz[k]["value"] = 0.0
+    assert not np.array_equal(xo, orc.mlem(ends, el_daz, 4, y, 6)[0])
+    A.close()
+
+
 FAMILIES = [("tiles", 4), ("tiles", 8), ("tiles", 16), ("dict", 8), ("dict", 16),
             ("dict", 24), ("dict", 32), ("dict", 48), ("dict", 64), ("dict512", 16),
             ("dict512", 24), ("dict512", 32), ("groups", 4), ("groups", 16), ("groups", 64)]
