@@ -112,21 +112,26 @@ def large_run(steps: int, fp64_iters: int) -> dict:
     return out
 
 
-def unrolled_run(A, prob, frames: int, K: int, checkpoint: int) -> dict:
+def unrolled_run(A, prob, frames: int, K: int, checkpoint: int, mode: str = "fast") -> dict:
     """Differentiable unrolled MLEM on the same matrix: `frames` sinograms
     with relaxed (real) counts, K iterations from x0 = sum(y)/sum(n), loss
     0.5 ||x_K - x_true||^2; times the forward pass and the reverse pass
-    (dL/dy, dL/dx0 with recomputation from checkpoints)."""
+    (dL/dy, dL/dx0 with recomputation from checkpoints). Algorithmic bytes of
+    one iteration over a block of W frames: the matrix read for both products
+    at 8 bytes per element each, and every vector run once (counts, f and r
+    per row; x read and written and the column sums per column, W*4 bytes
+    each); the reverse pass recomputes K iterations and runs K adjoint
+    iterations of the same two products."""
     import mlem_amd as M
     rng = np.random.default_rng(4000)
     Y = np.stack([rng.poisson(prob.ybar * s) + rng.uniform(0, 0.5, prob.ybar.size)
                   for s in rng.uniform(0.5, 2.0, frames)]).astype(np.float32)
-    U = M.Unrolled(A, Y, None, K, checkpoint_interval=checkpoint)  # warm-up
+    U = M.Unrolled(A, Y, None, K, checkpoint_interval=checkpoint, mode=mode)  # warm-up
     U.backward(U.xK - prob.x_true)
     U.close()
     M.device_sync()
     t0 = time.perf_counter()
-    U = M.Unrolled(A, Y, None, K, checkpoint_interval=checkpoint)
+    U = M.Unrolled(A, Y, None, K, checkpoint_interval=checkpoint, mode=mode)
     M.device_sync()
     t1 = time.perf_counter()
     gy, gx0 = U.backward(U.xK - prob.x_true)
@@ -134,8 +139,21 @@ def unrolled_run(A, prob, frames: int, K: int, checkpoint: int) -> dict:
     t2 = time.perf_counter()
     info = U.info()
     U.close()
-    return {"frames": frames, "K": K, **info,
-            "forward_ms": 1e3 * (t1 - t0), "backward_ms": 1e3 * (t2 - t1),
+    m = prob.matrix
+    W = info["block_width"]
+    blocks = -(-frames // W)
+    per_iter = blocks * (16 * m.nnz + 4 * W * (3 * m.rows + 3 * m.cols))
+    fwd_s, bwd_s = t1 - t0, t2 - t1
+    fwd_gbs = K * per_iter / fwd_s / 1e9
+    bwd_gbs = 2 * K * per_iter / bwd_s / 1e9
+    return {"mode": mode, "frames": frames, "K": K, **info,
+            "forward_ms": 1e3 * fwd_s, "backward_ms": 1e3 * bwd_s,
+            "frame_iterations_per_s": {"forward": frames * K / fwd_s,
+                                       "backward": frames * K / bwd_s},
+            "roofline": {"bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS,
+                         "algorithmic_bytes_per_iteration": per_iter,
+                         "forward": {"achieved": fwd_gbs, "frac": fwd_gbs / PEAK_HBM_GBS},
+                         "backward": {"achieved": bwd_gbs, "frac": bwd_gbs / PEAK_HBM_GBS}},
             "note": "host-to-device of y and device-to-host of x_K, dL/dy, dL/dx0 "
                     "included in the two times",
             "grad_finite": bool(np.all(np.isfinite(gy)) and np.all(np.isfinite(gx0)))}
@@ -488,7 +506,10 @@ def main() -> int:
     unrolled = None
     if args.unrolled_frames > 0 and args.precision == "fp32" and args.mode != "atomic":
         try:
-            unrolled = unrolled_run(A, prob, args.unrolled_frames, 20, 5)
+            unrolled = unrolled_run(A, prob, args.unrolled_frames, 20, 5, "fast")
+            ex = unrolled_run(A, prob, args.unrolled_frames, 20, 5, "exact")
+            unrolled["exact_mode"] = {k: ex[k] for k in ("forward_ms", "backward_ms",
+                                                         "frame_iterations_per_s")}
         except Exception as e:  # reported in the line, never hides the plain result
             unrolled = {"error": f"{type(e).__name__}: {e}"}
         print(f"[bench] unrolled: {unrolled}", file=sys.stderr, flush=True)

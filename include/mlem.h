@@ -379,6 +379,15 @@ typedef struct mlem_tape mlem_tape;
 MLEM_API int mlem_forward_k(mlem_matrix* m, uint32_t frames, const float* y, uint32_t rows,
                             const float* x0, uint32_t cols, int K, int checkpoint_interval,
                             float* xK, mlem_tape** tape);
+/* mlem_forward_k with the products' arithmetic chosen (mlem_forward_k is
+ * MLEM_MODE_EXACT): MLEM_MODE_EXACT sums every frame's products one by one in
+ * file element order (for integer counts x_K is bit-identical to mlem());
+ * MLEM_MODE_FAST runs the products through the matrix's panel tiling (the
+ * batched FAST kernels), fixed-order sums, deterministic for a given block
+ * width. mlem_backward_k runs in the tape's mode. No reference counterpart. */
+MLEM_API int mlem_forward_k_mode(mlem_matrix* m, uint32_t frames, const float* y,
+                                 uint32_t rows, const float* x0, uint32_t cols, int K,
+                                 int checkpoint_interval, int mode, float* xK, mlem_tape** tape);
 /* grad_xK: frames*cols dL/dx_K; grad_y (frames*rows) and grad_x0
  * (frames*cols) may be NULL. May be called more than once per tape. */
 MLEM_API int mlem_backward_k(mlem_tape* t, const float* grad_xK, float* grad_y,

@@ -437,10 +437,14 @@ class Unrolled:
 
     u = Unrolled(A, y, x0, K)       runs the forward pass; u.xK is x_K
     gy, gx0 = u.backward(dL/dx_K)   reverse mode (recomputes from checkpoints)
+
+    mode="exact": every frame's products summed one by one in file element
+    order (x_K bit-identical to mlem() for integer counts); mode="fast": the
+    products through the matrix's panel tiling, fixed-order sums.
     """
 
     def __init__(self, matrix: Csr4Matrix, y, x0=None, K: int = 20, *,
-                 checkpoint_interval: int = 0, keep_tape: bool = True):
+                 checkpoint_interval: int = 0, keep_tape: bool = True, mode="exact"):
         y = np.ascontiguousarray(np.atleast_2d(y), np.float32)
         frames, rows = y.shape
         if rows != matrix.rows():
@@ -453,9 +457,9 @@ class Unrolled:
         self.matrix, self.frames, self.K = matrix, frames, int(K)
         self.xK = np.zeros((frames, cols), np.float32)
         t = C.c_void_p()
-        check(lib().mlem_forward_k(matrix.handle, frames, _ptr(y), rows, _ptr(x0), cols,
-                                   self.K, int(checkpoint_interval), _ptr(self.xK),
-                                   C.byref(t) if keep_tape else None))
+        check(lib().mlem_forward_k_mode(matrix.handle, frames, _ptr(y), rows, _ptr(x0), cols,
+                                        self.K, int(checkpoint_interval), _mode(mode),
+                                        _ptr(self.xK), C.byref(t) if keep_tape else None))
         self._t = t if keep_tape else None
 
     def backward(self, grad_xK):

@@ -116,6 +116,8 @@ SIGNATURES = {
     "mlem_batch_destroy": (None, [_P]),
     "mlem_forward_k": (C.c_int, [_P, _u32, _P, _u32, _P, _u32, C.c_int, C.c_int, _P,
                                  C.POINTER(_P)]),
+    "mlem_forward_k_mode": (C.c_int, [_P, _u32, _P, _u32, _P, _u32, C.c_int, C.c_int, C.c_int,
+                                      _P, C.POINTER(_P)]),
     "mlem_backward_k": (C.c_int, [_P, _P, _P, _P]),
     "mlem_tape_info": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                  C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),

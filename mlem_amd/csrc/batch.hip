@@ -3,14 +3,21 @@
 //
 // Frames are grouped into blocks of W (4, 8, 16 or 32) right-hand sides. A
 // block's image, counts and ratios are stored interleaved per voxel / row
-// (X[col*W + f]), so the W lanes that own one voxel's frames gather a whole
+// (X[col*W + f]), so the lanes that own one voxel's frames gather a whole
 // contiguous W*4-byte run (one 128-byte line at W = 32) and every matrix
-// record streamed from HBM serves W frames. Each 256-lane block walks whole
-// tiles of rows (CSC columns), stages the tile's 8-byte records in LDS with
-// coalesced loads, and gives every row (column) to a group of W lanes, lane f
-// accumulating frame f sequentially in file element order with separate
-// multiply and add: per frame, the result is bit-identical to the reference's
-// sequential driver (mlem_seq/seqcsr4mlem.cpp:69-134).
+// record streamed from HBM serves W frames.
+//
+// MLEM_MODE_FAST (the default): a block's products run through the matrix's
+// two-dimensional tiling (panel.hip): partial sums per (row, panel) segment
+// and per (panel, row chunk, voxel) group, added in a fixed order by the
+// epilogue kernels k_tiled_ratio (ratio and statistics) and k_tiled_update
+// (masked update); deterministic, not the reference's summation order.
+// MLEM_MODE_EXACT: each 256-lane block walks whole tiles of rows (CSC
+// columns), stages the tile's 8-byte records in LDS with coalesced loads, and
+// gives every row (column) to a group of W lanes, lane f accumulating frame f
+// sequentially in file element order with separate multiply and add: per
+// frame, the result is bit-identical to the reference's sequential driver
+// (mlem_seq/seqcsr4mlem.cpp:69-134).
 //
 // Per frame and iteration, kernels after the forward pass produce the FP64
 // Poisson log-likelihood sum_{f_i != 0} (y_i log f_i - f_i) and sum_i f_i with
@@ -18,9 +25,9 @@
 // is spent or the relative log-likelihood change drops below the tolerance,
 // freezes it, and repacks the remaining frames into fewer / narrower blocks.
 // An autotuned batch may also give a frame a block of its own (width 1): it
-// then runs on the single-image kernels in EXACT mode, whose arithmetic per
-// frame is the same sequential fp32 one, so the few frames left at the end
-// of a dynamic series do not pay for a whole block each.
+// then runs on the single-image kernels in the batch's mode (in EXACT mode
+// the same sequential fp32 arithmetic per frame), so the few frames left at
+// the end of a dynamic series do not pay for a whole block each.
 #include <algorithm>
 #include <cmath>
 #include <cstring>

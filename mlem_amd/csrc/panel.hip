@@ -1,14 +1,17 @@
 // panel.hip — products of the system matrix with blocks of W interleaved
 // right-hand sides through its two-dimensional tiling (panels.hpp), gfx950.
 //
-// Forward, f = A X: a workgroup takes a tile of segments of one panel (whole
-// (row, panel) pieces of rows), stages their values and 16-bit in-panel
-// columns in LDS with coalesced loads and gives every segment to a group of W
-// lanes; lane f gathers X[(panel start + column) * W + f] (one W*4-byte run
-// per element, from the panel's 2 MB slice of X at W = 32) and writes its
-// fused-multiply-add dot product to the segment's slot. Backward, u = A^T R:
-// the same over the (panel, row chunk, voxel) groups, gathering R[row * W + f]
-// from the chunk's rows. Workgroup b runs on XCD b % 8 and XCD x walks the
+// Forward, f = A X: a workgroup takes a tile of segments of one panel
+// ((row, panel) pieces of rows, at most kMaxSegment elements each), stages
+// their values and 16-bit in-panel columns in LDS with coalesced loads and
+// gives every segment to 8 lanes: W/4 quads of frames times 8/(W/4) shares of
+// the elements (every 8/(W/4)-th element). Per element a lane gathers one
+// 16-byte quad of X[(panel start + column) * W + ...] (the quads of the W/4
+// lanes make the element's W*4-byte run, from the panel's 2 MB slice of X at
+// W = 32) and accumulates with fused multiply-adds; a butterfly over the
+// lanes adds the shares and the segment's sums go to its slot. Backward,
+// u = A^T R: the same over the (panel, row chunk, voxel) groups, gathering
+// R[row * W + ...] from the chunk's rows. Workgroup b runs on XCD b % 8 and XCD x walks the
 // x-th contiguous eighth of the tiles, which are stored panel after panel
 // (chunk after chunk): the 256 workgroups of one XCD work through the same
 // panel (chunk) together, so its gathered slice stays in that XCD's L2.

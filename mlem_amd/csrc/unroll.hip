@@ -14,10 +14,16 @@
 //   gu = g x_k/n,  g <- g u_k/n               (elementwise, n = 0: drop /n)
 //   gr = A gu:      dL/dy += gr/f_k,  gf = -gr r_k/f_k   (0 where f_k = 0)
 //   g += A^T gf
-// and dL/dx0 = g. Every product is the RHS-block SpMM of spmm.hpp: the
-// batch's frames are lanes of one block (W = 4..32), so each matrix record
-// read from HBM serves every frame, and each frame's sums are sequential
-// fp32 (deterministic; the same for any batch composition).
+// and dL/dx0 = g. The batch's frames are lanes of one block (W = 4..32), so
+// each matrix element read from HBM serves every frame. MLEM_MODE_EXACT:
+// every product is the RHS-block SpMM of spmm.hpp, each frame's sums
+// sequential fp32 (deterministic; the same for any batch composition; for
+// integer counts the forward pass is bit-identical to mlem()).
+// MLEM_MODE_FAST: every product runs through the matrix's panel tiling
+// (panel.hip): partial sums per (row, panel) segment or (panel, row chunk,
+// voxel) group, added in a fixed order by k_slot_sums before the epilogue;
+// column vectors are then held in panel order. Deterministic for a given
+// block width.
 //
 // Memory per block of W frames: ceil(K/C) checkpoints of cols*W floats plus
 // C iterations of (2 rows + 2 cols)*W floats; C = round(sqrt(K)) by default.
@@ -113,6 +119,23 @@ __global__ __launch_bounds__(kBlock) void k_spmm(const uint32_t* __restrict__ pt
   spmm::walk<W>(ptr, el, tiles, ntiles, v, epi);
 }
 
+// Fixed-order sums of a tiled product's partials: lane (i, l) adds
+// part[s*W + l] over the slots s in [first[i], first[i+1]) (the slots of row
+// i in panel order, or of panel-order column i in chunk order) and hands the
+// sum to the epilogue.
+template <int W, class Epi>
+__global__ __launch_bounds__(kBlock) void k_slot_sums(const uint32_t* __restrict__ first,
+                                                      uint32_t n, const float* __restrict__ part,
+                                                      Epi epi) {
+  const uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
+  if (i >= (uint64_t)n * W) return;
+  const uint32_t r = (uint32_t)(i / W), l = (uint32_t)(i % W);
+  const uint32_t s0 = first[r], s1 = first[r + 1];
+  float acc = 0.f;
+  for (uint32_t k = s0; k < s1; ++k) acc = acc + part[(size_t)k * W + l];
+  epi(r, l, acc);
+}
+
 // gu = g x/n, g <- g u/n (n = 0: without /n), over cols*W.
 __global__ __launch_bounds__(kBlock) void k_adj_update(float* __restrict__ G,
                                                        float* __restrict__ GU,
@@ -204,6 +227,14 @@ struct mlem_tape {
   mlem_matrix* mat = nullptr;
   uint32_t F = 0;
   int W = 4, K = 0, C = 1, nseg = 0;
+  int mode = MLEM_MODE_EXACT;
+  // MLEM_MODE_FAST: the matrix's tiling, partial-sum buffers (slots x W),
+  // and the column order (panel order) and column sums the vectors use
+  const PanelFormat* P = nullptr;
+  float* fpart = nullptr;
+  float* upart = nullptr;
+  const uint32_t* colmap = nullptr;
+  const float* norm = nullptr;
   uint32_t grid = 0;
   uint2* tiles_f = nullptr;
   uint2* tiles_b = nullptr;
@@ -216,29 +247,43 @@ struct mlem_tape {
     blocks.clear();
     if (tiles_f) (void)hipFree(tiles_f);
     if (tiles_b) (void)hipFree(tiles_b);
+    if (fpart) (void)hipFree(fpart);
+    if (upart) (void)hipFree(upart);
   }
 };
 
 namespace {
 
+// One product with the block's W vectors v: A v (csc = false, epilogue per
+// row) or A^T v (csc = true, epilogue per column).
 template <int W, class Epi>
 void spmm_launch(mlem_tape* T, bool csc, const float* v, const Epi& epi) {
   const DevMatrix& m = T->mat->m;
-  if (csc)
+  if (T->mode == MLEM_MODE_FAST) {
+    const PanelFormat& P = *T->P;
+    const uint32_t n = csc ? m.cols : m.rows;
+    float* part = csc ? T->upart : T->fpart;
+    if (csc)
+      launch_panel_bwd<W>(P, v, part, T->grid, m.stream);
+    else
+      launch_panel_fwd<W>(P, v, part, T->grid, m.stream);
+    hipLaunchKernelGGL((k_slot_sums<W, Epi>), dim3(blocks_for((uint64_t)n * W)), dim3(kBlock), 0,
+                       m.stream, csc ? P.colslot : P.rowslot, n, part, epi);
+  } else if (csc) {
     hipLaunchKernelGGL((k_spmm<W, Epi>), dim3(T->grid), dim3(kBlock), 0, m.stream, m.colptr, m.csc,
                        T->tiles_b, T->nt_b, v, epi);
-  else
+  } else {
     hipLaunchKernelGGL((k_spmm<W, Epi>), dim3(T->grid), dim3(kBlock), 0, m.stream, m.rowptr, m.csr,
                        T->tiles_f, T->nt_f, v, epi);
+  }
   ++T->spmm_launches;
 }
 
 // One MLEM iteration of a block: x -> xn, keeping f, r (and u if U).
 template <int W>
 void step(mlem_tape* T, Block& b, const float* x, float* xn, float* F, float* R, float* U) {
-  const DevMatrix& m = T->mat->m;
   spmm_launch<W>(T, false, x, StepFwd<W>{b.Y.p, F, R});
-  spmm_launch<W>(T, true, R, StepBwd<W>{m.norm, x, xn, U});
+  spmm_launch<W>(T, true, R, StepBwd<W>{T->norm, x, xn, U});
 }
 
 template <int W>
@@ -283,7 +328,7 @@ void backward_block(mlem_tape* T, Block& b) {
     for (int k = e - 1; k >= s; --k) {
       const int i = k - s;
       hipLaunchKernelGGL(k_adj_update, dim3(blocks_for(cw)), dim3(kBlock), 0, m.stream, b.G.p,
-                         b.GU.p, xs[i], b.su.p + (uint64_t)i * cw, m.norm, W, cw);
+                         b.GU.p, xs[i], b.su.p + (uint64_t)i * cw, T->norm, W, cw);
       spmm_launch<W>(T, false, b.GU.p,
                      AdjFwd<W>{b.sf.p + (uint64_t)i * rw, b.sr.p + (uint64_t)i * rw, b.GY.p,
                                b.GF.p});
@@ -311,6 +356,13 @@ extern "C" {
 int mlem_forward_k(mlem_matrix* h, uint32_t frames, const float* y, uint32_t rows,
                    const float* x0, uint32_t cols, int K, int checkpoint_interval, float* xK,
                    mlem_tape** tape) {
+  return mlem_forward_k_mode(h, frames, y, rows, x0, cols, K, checkpoint_interval,
+                             MLEM_MODE_EXACT, xK, tape);
+}
+
+int mlem_forward_k_mode(mlem_matrix* h, uint32_t frames, const float* y, uint32_t rows,
+                        const float* x0, uint32_t cols, int K, int checkpoint_interval, int mode,
+                        float* xK, mlem_tape** tape) {
   return guarded([&] {
     if (tape) *tape = nullptr;
     if (!h || !y || frames == 0) raise(MLEM_EINVAL, "bad argument");
@@ -322,9 +374,12 @@ int mlem_forward_k(mlem_matrix* h, uint32_t frames, const float* y, uint32_t row
     if (m.streamed) raise(MLEM_EINVAL, "unrolled MLEM needs a matrix held in HBM");
     if (!m.csc) raise(MLEM_ESTATE, "unrolled MLEM needs the CSC transpose");
     if (m.vk != kF32) raise(MLEM_EINVAL, "unrolled MLEM needs fp32 matrix values");
+    if (mode != MLEM_MODE_EXACT && mode != MLEM_MODE_FAST)
+      raise(MLEM_EINVAL, "unrolled MLEM runs MLEM_MODE_EXACT or MLEM_MODE_FAST");
     std::unique_ptr<mlem_tape> T(new mlem_tape());
     T->mat = h;
     T->F = frames;
+    T->mode = mode;
     T->K = K;
     T->C = checkpoint_interval > 0 ? std::min(checkpoint_interval, K)
                                    : std::max(1, (int)std::lround(std::sqrt((double)K)));
@@ -333,25 +388,35 @@ int mlem_forward_k(mlem_matrix* h, uint32_t frames, const float* y, uint32_t row
     while (W < 32 && W < (int)frames) W *= 2;
     T->W = W;
     T->grid = (uint32_t)std::max(8, 8 * m.cus);
+    T->colmap = m.colmap;
+    T->norm = m.norm;
+    if (mode == MLEM_MODE_FAST) {
+      T->P = &ensure_panels(m);
+      T->colmap = T->P->pc_of_file;
+      T->norm = T->P->norm;
+      MLEM_HIP(hipMalloc(&T->fpart, 4ull * std::max<uint64_t>(T->P->nseg, 1) * W));
+      MLEM_HIP(hipMalloc(&T->upart, 4ull * std::max<uint64_t>(T->P->ngroups, 1) * W));
+    }
     prepare_tiles(m, m.stream);
-    const std::vector<uint32_t> tf = spmm::partition(m.h_rowptr, kTile),
-                                tb = spmm::partition(m.h_colptr, kTile);
-    std::vector<uint2> df(tf.size()), db(tb.size());
-    for (size_t i = 0; i < tf.size(); ++i) df[i] = make_uint2(tf[i], m.h_rowptr[tf[i]]);
-    for (size_t i = 0; i < tb.size(); ++i) db[i] = make_uint2(tb[i], m.h_colptr[tb[i]]);
-    T->nt_f = (uint32_t)tf.size() - 1;
-    T->nt_b = (uint32_t)tb.size() - 1;
-    MLEM_HIP(hipMalloc(&T->tiles_f, sizeof(uint2) * df.size()));
-    MLEM_HIP(hipMalloc(&T->tiles_b, sizeof(uint2) * db.size()));
-    MLEM_HIP(hipMemcpyAsync(T->tiles_f, df.data(), sizeof(uint2) * df.size(),
-                            hipMemcpyHostToDevice, m.stream));
-    MLEM_HIP(hipMemcpyAsync(T->tiles_b, db.data(), sizeof(uint2) * db.size(),
-                            hipMemcpyHostToDevice, m.stream));
+    if (mode == MLEM_MODE_EXACT) {  // record tiles of the sequential SpMM walker
+      const std::vector<uint32_t> tf = spmm::partition(m.h_rowptr, kTile),
+                                  tb = spmm::partition(m.h_colptr, kTile);
+      std::vector<uint2> df(tf.size()), db(tb.size());
+      for (size_t i = 0; i < tf.size(); ++i) df[i] = make_uint2(tf[i], m.h_rowptr[tf[i]]);
+      for (size_t i = 0; i < tb.size(); ++i) db[i] = make_uint2(tb[i], m.h_colptr[tb[i]]);
+      T->nt_f = (uint32_t)tf.size() - 1;
+      T->nt_b = (uint32_t)tb.size() - 1;
+      MLEM_HIP(hipMalloc(&T->tiles_f, sizeof(uint2) * df.size()));
+      MLEM_HIP(hipMalloc(&T->tiles_b, sizeof(uint2) * db.size()));
+      MLEM_HIP(hipMemcpyAsync(T->tiles_f, df.data(), sizeof(uint2) * df.size(),
+                              hipMemcpyHostToDevice, m.stream));
+      MLEM_HIP(hipMemcpyAsync(T->tiles_b, db.data(), sizeof(uint2) * db.size(),
+                              hipMemcpyHostToDevice, m.stream));
+    }
 
     const uint64_t cw = (uint64_t)m.cols * W, rw = (uint64_t)m.rows * W;
     DevBuf stage;  // one frame's file-order vector
     stage.alloc(std::max<uint64_t>(m.rows, m.cols));
-    std::vector<float> x0f;
     const size_t nb = (frames + W - 1) / W;
     for (size_t bi = 0; bi < nb; ++bi) {
       std::unique_ptr<Block> b(new Block());
@@ -385,17 +450,21 @@ int mlem_forward_k(mlem_matrix* h, uint32_t frames, const float* y, uint32_t row
           MLEM_HIP(hipMemcpyAsync(stage.p, x0 + f * cols, 4ull * cols, hipMemcpyHostToDevice,
                                   m.stream));
         } else {
-          // x0 = (float)(sum y / sum n) with double sums (seqcsr4mlem.cpp:52-66)
-          double sy = 0.0;
-          for (uint32_t i = 0; i < rows; ++i) sy += yf[i];
-          x0f.assign(cols, (float)(sy / m.sumnorm));
-          MLEM_HIP(hipMemcpyAsync(stage.p, x0f.data(), 4ull * cols, hipMemcpyHostToDevice,
-                                  m.stream));
+          // x0 = (float)(sum y / sum n) with double sums (seqcsr4mlem.cpp:52-66),
+          // y taken in eight interleaved partial sums (exact for integer
+          // counts, as the reference's single running sum is)
+          double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+          uint32_t i = 0;
+          for (; i + 8 <= rows; i += 8)
+            for (int j = 0; j < 8; ++j) acc[j] += yf[i + j];
+          for (; i < rows; ++i) acc[0] += yf[i];
+          const double sy = ((acc[0] + acc[1]) + (acc[2] + acc[3])) +
+                            ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+          launch_fill(stage.p, (float)(sy / m.sumnorm), cols, m.stream);
         }
+        // the staging buffer is reused by the next frame in stream order
         hipLaunchKernelGGL(k_cols_in, dim3(blocks_for(cols)), dim3(kBlock), 0, m.stream,
-                           b->ckpt.p, W, l, stage.p, m.colmap, cols);
-        // the staging buffer is reused by the next frame: keep order
-        MLEM_HIP(hipStreamSynchronize(m.stream));
+                           b->ckpt.p, W, l, stage.p, T->colmap, cols);
       }
       check_launch("unrolled input launch");
       T->blocks.push_back(std::move(b));
@@ -406,10 +475,9 @@ int mlem_forward_k(mlem_matrix* h, uint32_t frames, const float* y, uint32_t row
         for (int l = 0; l < W; ++l) {
           if (b->frames[l] < 0) continue;
           hipLaunchKernelGGL(k_cols_out, dim3(blocks_for(cols)), dim3(kBlock), 0, m.stream, b->xK,
-                             W, l, stage.p, m.colmap, cols);
+                             W, l, stage.p, T->colmap, cols);
           MLEM_HIP(hipMemcpyAsync(xK + (size_t)b->frames[l] * cols, stage.p, 4ull * cols,
                                   hipMemcpyDeviceToHost, m.stream));
-          MLEM_HIP(hipStreamSynchronize(m.stream));
         }
     }
     MLEM_HIP(hipStreamSynchronize(m.stream));
@@ -426,7 +494,7 @@ int mlem_backward_k(mlem_tape* T, const float* grad_xK, float* grad_y, float* gr
     const uint32_t rows = m.rows, cols = m.cols;
     const int W = T->W;
     const uint64_t cw = (uint64_t)cols * W;
-    DevBuf stage;
+    DevBuf stage;  // one frame's file-order vector, reused in stream order
     stage.alloc(std::max<uint64_t>(rows, cols));
     for (auto& b : T->blocks) {
       hipLaunchKernelGGL(k_zero, dim3(std::min<uint32_t>(blocks_for(cw), 4096)), dim3(kBlock), 0,
@@ -436,8 +504,7 @@ int mlem_backward_k(mlem_tape* T, const float* grad_xK, float* grad_y, float* gr
         MLEM_HIP(hipMemcpyAsync(stage.p, grad_xK + (size_t)b->frames[l] * cols, 4ull * cols,
                                 hipMemcpyHostToDevice, m.stream));
         hipLaunchKernelGGL(k_cols_in, dim3(blocks_for(cols)), dim3(kBlock), 0, m.stream, b->G.p,
-                           W, l, stage.p, m.colmap, cols);
-        MLEM_HIP(hipStreamSynchronize(m.stream));
+                           W, l, stage.p, T->colmap, cols);
       }
       dispatch(W, [&](auto w) { backward_block<decltype(w)::value>(T, *b); });
       for (int l = 0; l < W; ++l) {
@@ -448,14 +515,12 @@ int mlem_backward_k(mlem_tape* T, const float* grad_xK, float* grad_y, float* gr
                              b->GY.p, W, l, stage.p, m.rowold, rows);
           MLEM_HIP(hipMemcpyAsync(grad_y + (size_t)f * rows, stage.p, 4ull * rows,
                                   hipMemcpyDeviceToHost, m.stream));
-          MLEM_HIP(hipStreamSynchronize(m.stream));
         }
         if (grad_x0) {
           hipLaunchKernelGGL(k_cols_out, dim3(blocks_for(cols)), dim3(kBlock), 0, m.stream,
-                             b->G.p, W, l, stage.p, m.colmap, cols);
+                             b->G.p, W, l, stage.p, T->colmap, cols);
           MLEM_HIP(hipMemcpyAsync(grad_x0 + (size_t)f * cols, stage.p, 4ull * cols,
                                   hipMemcpyDeviceToHost, m.stream));
-          MLEM_HIP(hipStreamSynchronize(m.stream));
         }
       }
     }

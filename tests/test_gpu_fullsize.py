@@ -263,6 +263,38 @@ def test_pet_64_frame_dynamic_workload(pet, pet_frames, mode):
     B.close()
 
 
+# ---- unrolled MLEM at the size of the differentiable workload ---------------
+# The pet matrix, K = 20 unrolled iterations over a batch of 8 relaxed-count
+# sinograms, checkpoints every 5 iterations, loss 0.5 ||x_K - x_true||^2:
+# x_K and both gradients of one frame against the FP64 oracle's forward and
+# reverse mode (about 35 s of single-threaded FP64 per frame).
+
+def _rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("mode,frame", [("fast", 7), ("exact", 0)])
+def test_pet_unrolled_batch8_k20(pet, mode, frame):
+    p, A = pet
+    m = p.matrix
+    rng = np.random.default_rng(4000)
+    Y = np.stack([rng.poisson(p.ybar * s) + rng.uniform(0, 0.5, p.ybar.size)
+                  for s in rng.uniform(0.5, 2.0, 8)]).astype(np.float32)
+    u = M.Unrolled(A, Y, None, K=20, checkpoint_interval=5, mode=mode)
+    assert u.info()["block_width"] == 8 and u.info()["checkpoints"] == 4
+    g = (u.xK - p.x_true).astype(np.float32)
+    gy, gx0 = u.backward(g)
+    u.close()
+    assert np.all(np.isfinite(gy)) and np.all(np.isfinite(gx0))
+    n64 = float(np.sum(M.calcColumnSums(A).astype(np.float64)))
+    x0 = np.full(m.cols, np.float32(np.sum(Y[frame].astype(np.float64)) / n64))
+    xK64, gy64, gx064 = orc.f64_unroll(m.rowidx, m.elements, m.cols, Y[frame], x0, 20,
+                                       gK=g[frame].astype(np.float64))
+    assert rel_linf(u.xK[frame], xK64) < 1e-4
+    assert _rel_l2(gy[frame], gy64) < 1e-4
+    assert _rel_l2(gx0[frame], gx064) < 1e-4
+
+
 # ---- 16-bit matrix values on the 192x192x64 / 4,194,304-line matrix ---------
 
 @pytest.mark.parametrize("kind", ["fp16", "bf16"])

@@ -65,14 +65,15 @@ def test_forward_default_x0_matches_mlem(tiny_u):
     assert np.array_equal(u.xK[0], orc.mlem(m.rowidx, m.elements, m.cols, p.y, 5)[0])
 
 
+@pytest.mark.parametrize("mode", ["exact", "fast"])
 @pytest.mark.parametrize("K,C", [(6, 2), (6, 0), (5, 1), (5, 5), (7, 3)])
-def test_gradient_vs_fp64_oracle(tiny_u, K, C):
+def test_gradient_vs_fp64_oracle(tiny_u, K, C, mode):
     p, A = tiny_u
     m = p.matrix
     Y = _frames(p, 5, 2)
     rng = np.random.default_rng(3)
     X0 = rng.uniform(0.5, 1.5, (5, m.cols))
-    u = M.Unrolled(A, Y, X0, K=K, checkpoint_interval=C)
+    u = M.Unrolled(A, Y, X0, K=K, checkpoint_interval=C, mode=mode)
     info = u.info()
     Ce = C if C else int(round(np.sqrt(K)))
     assert info["checkpoint_interval"] == Ce and info["checkpoints"] == -(-K // Ce)
@@ -119,6 +120,33 @@ def test_gradient_independent_of_checkpointing_and_batch(tiny_u):
     u.close()
 
 
+def test_fast_gradient_deterministic_and_independent_of_checkpointing(tiny_u):
+    # the tiled products sum in a fixed order: for one block width, x_K and
+    # the gradients are the same bits whatever the checkpoint interval, and
+    # a repeated backward pass gives them again
+    p, A = tiny_u
+    m = p.matrix
+    Y = _frames(p, 3, 4)
+    rng = np.random.default_rng(5)
+    X0 = rng.uniform(0.5, 1.5, (3, m.cols)).astype(np.float32)
+    G = rng.standard_normal((3, m.cols)).astype(np.float32)
+    ref = None
+    for C in (1, 2, 3, 8):
+        u = M.Unrolled(A, Y, X0, K=8, checkpoint_interval=C, mode="fast")
+        gy, gx0 = u.backward(G)
+        if ref is None:
+            ref = (u.xK.copy(), gy, gx0)
+            gy2, gx02 = u.backward(G)
+            assert np.array_equal(gy, gy2) and np.array_equal(gx0, gx02)
+        else:
+            assert np.array_equal(u.xK, ref[0])
+            assert np.array_equal(gy, ref[1]) and np.array_equal(gx0, ref[2]), C
+        u.close()
+    ex = M.Unrolled(A, Y, X0, K=8, checkpoint_interval=2)
+    assert rel_linf(ref[0], ex.xK) < 1e-5      # FAST next to the sequential sums
+    ex.close()
+
+
 def test_finite_differences_and_homogeneity(tiny_u):
     p, A = tiny_u
     m = p.matrix
@@ -152,14 +180,15 @@ def test_finite_differences_and_homogeneity(tiny_u):
     assert abs(np.dot(y32, gy) - want) <= 1e-4 * abs(want)
 
 
-def test_small_batch8_k20():
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_small_batch8_k20(mode):
     # 32^3 workload, batch of 8 relaxed-count sinograms, K = 20, checkpoints
     # every 5 iterations
     p = synth.make_problem("small")
     m = p.matrix
     A = _open(m)
     Y = _frames(p, 8, 7)
-    u = M.Unrolled(A, Y, None, K=20, checkpoint_interval=5)
+    u = M.Unrolled(A, Y, None, K=20, checkpoint_interval=5, mode=mode)
     assert u.info()["block_width"] == 8
     g = (u.xK - p.x_true).astype(np.float32)
     gy, gx0 = u.backward(g)
@@ -182,6 +211,8 @@ def test_unrolled_argument_errors(tiny_u):
         M.Unrolled(A, p.y[None, :-1].astype(np.float32))
     with pytest.raises(M.MlemError):
         M.Unrolled(A, p.y[None, :].astype(np.float32), K=0)
+    with pytest.raises(M.MlemError):
+        M.Unrolled(A, p.y[None, :].astype(np.float32), K=2, mode="atomic")
     u = M.Unrolled(A, p.y[None, :].astype(np.float32), K=2, keep_tape=False)
     with pytest.raises(RuntimeError):
         u.backward(u.xK)
