@@ -8,10 +8,10 @@
 // record streamed from HBM serves W frames.
 //
 // MLEM_MODE_FAST (the default): a block's products run through the matrix's
-// two-dimensional tiling (panel.hip): partial sums per (row, panel) segment
-// and per (panel, row chunk, voxel) group, added in a fixed order by the
-// epilogue kernels k_tiled_ratio (ratio and statistics) and k_tiled_update
-// (masked update); deterministic, not the reference's summation order.
+// box tiling (panel.hip): sums per (row, box) piece, added in a fixed order
+// by the epilogue kernel k_tiled_ratio (ratio and statistics), and complete
+// per-voxel backprojections, applied by k_tiled_update (masked update);
+// deterministic, not the reference's summation order.
 // MLEM_MODE_EXACT: each 256-lane block walks whole tiles of rows (CSC
 // columns), stages the tile's 8-byte records in LDS with coalesced loads, and
 // gives every row (column) to a group of W lanes, lane f accumulating frame f
@@ -241,34 +241,38 @@ __device__ __forceinline__ void stat1(int32_t y, float f, double& ll, double& to
   if (f > 0.f) ll += (y != 0 ? (double)y * log((double)f) : 0.0) - (double)f;
 }
 
-// Epilogue of the tiled forward product (FAST mode): a row's segment slots
-// added in panel order, then ratio r = (f != 0) ? y/f : 0 and the frames'
+// Epilogue of the tiled forward product (FAST mode): a row's piece sums
+// added in box order (read through rowpiece where the forward writes them
+// box by box), then ratio r = (f != 0) ? y/f : 0 and the frames'
 // FP64 statistics (row groups g, g + groups, ...); one partial per workgroup
 // and frame.
 // (64 VGPRs: the grid of 8 workgroups per CU is resident at once)
 template <int W>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_tiled_ratio(const uint32_t* __restrict__ rowslot,
+                                                        const uint32_t* __restrict__ rowpiece,
                                                         const float* __restrict__ part,
                                                         const int32_t* __restrict__ Y,
                                                         float* __restrict__ R, uint32_t rows,
                                                         double2* __restrict__ partials) {
   constexpr uint32_t L = Quads<W>::L, G = Quads<W>::G;
+  constexpr bool kByBox = panel_slots_by_box(W);  // slots gathered through rowpiece
   const uint32_t q = threadIdx.x % L, g = threadIdx.x / L;
   const uint32_t stride = gridDim.x * G;
   const float4* __restrict__ p4 = reinterpret_cast<const float4*>(part);
   const int4* __restrict__ y4 = reinterpret_cast<const int4*>(Y);
   float4* __restrict__ r4 = reinterpret_cast<float4*>(R);
+  auto slot = [&](uint32_t k) { return kByBox ? rowpiece[k] : k; };
   double ll[4] = {0.0, 0.0, 0.0, 0.0}, tot[4] = {0.0, 0.0, 0.0, 0.0};
   for (uint32_t row = blockIdx.x * G + g; row < rows; row += stride) {
     uint32_t s = rowslot[row];
     const uint32_t e = rowslot[row + 1];
     float4 F = make_float4(0.f, 0.f, 0.f, 0.f);
     for (; s + 2 <= e; s += 2) {
-      const float4 a0 = p4[(size_t)s * L + q], a1 = p4[(size_t)(s + 1) * L + q];
+      const float4 a0 = p4[(size_t)slot(s) * L + q], a1 = p4[(size_t)slot(s + 1) * L + q];
       add4(F, a0);
       add4(F, a1);
     }
-    if (s < e) add4(F, p4[(size_t)s * L + q]);
+    if (s < e) add4(F, p4[(size_t)slot(s) * L + q]);
     const size_t i = (size_t)row * L + q;
     const int4 y = y4[i];
     r4[i] = make_float4(ratio1(y.x, F.x), ratio1(y.y, F.y), ratio1(y.z, F.z), ratio1(y.w, F.w));
@@ -539,7 +543,8 @@ void launch_tiled_block(mlem_batch* B, RhsBlock& b, hipStream_t s) {
   const DevMatrix& m = B->mat->m;
   const PanelFormat& P = *B->P;
   launch_panel_fwd<W>(P, b.X, B->fpart, B->grid, s);
-  hipLaunchKernelGGL((k_tiled_ratio<W>), dim3(B->grid), dim3(kBlock), 0, s, P.rowslot, B->fpart,
+  hipLaunchKernelGGL((k_tiled_ratio<W>), dim3(B->grid), dim3(kBlock), 0, s, P.rowslot,
+                     P.rowpiece, B->fpart,
                      b.Y, b.R, m.rows, b.partials);
   hipLaunchKernelGGL((k_batch_stats_reduce<W>), dim3(1), dim3(kReduceLanes), 0, s, b.partials,
                      B->grid, b.counter, b.results, kRing);

@@ -149,25 +149,31 @@ struct StreamedMatrix {
 // of those products index columns in panel order.
 struct PanelFormat {
   uint32_t rows = 0, cols = 0;
-  uint32_t npanels = 0, nchunks = 0;
-  uint64_t nseg = 0, ngroups = 0;      // forward segments, backward groups (slots)
+  uint32_t npanels = 0, nchunks = 0;   // boxes, backward chunks
+  uint32_t nfchunks = 0;               // forward chunks
+  uint64_t nseg = 0, ngroups = 0;      // forward slots (pieces), backward slots (voxels)
   uint32_t* pc_of_file = nullptr;      // [cols] panel-order column of each file column
   uint32_t* pc_of_dev = nullptr;       // [cols] ... of each device column
   uint32_t* dev_of_pc = nullptr;       // [cols] device column of each panel-order column
   float* norm = nullptr;               // [cols] column sums in panel order
-  uint32_t* fseg = nullptr;            // [nseg+1] first element of each segment
-  uint32_t* fslot = nullptr;           // [nseg] row-major slot of each segment
+  uint32_t* pstart = nullptr;          // [npanels+1] first panel-order column of each box
+  uint32_t* border = nullptr;          // [npanels] launch order (largest box first)
+  uint32_t* pseg = nullptr;            // [npanels+1] first piece of each box
+  uint32_t* fseg = nullptr;            // [nseg+1] first element of each piece (forward order)
+  uint32_t* fslot = nullptr;           // [nseg] row-major slot of each piece
   float* fval = nullptr;               // [nnz]
-  uint16_t* fcol = nullptr;            // [nnz] column inside the panel
+  uint16_t* fcol = nullptr;            // [nnz] column inside the box
   uint32_t* rowslot = nullptr;         // [rows+1] first slot of each device row
-  uint4* ftiles = nullptr;             // [nftiles+1] {first segment, first element,
-  uint32_t nftiles = 0;                //  first panel-order column of the panel, 0}
-  uint32_t* bseg = nullptr;            // [ngroups+1] first entry of each group
-  uint32_t* bslot = nullptr;           // [ngroups] voxel-major slot of each group
-  uint2* bent = nullptr;               // [nnz] {device row, value bits}
-  uint32_t* colslot = nullptr;         // [cols+1] first slot of each panel-order column
-  uint4* btiles = nullptr;             // [nbtiles+1] {first group, first entry, 0, 0}
-  uint32_t nbtiles = 0;
+  uint32_t* rowpiece = nullptr;        // [nseg] forward-order piece of each row-major slot
+  uint32_t* pfchunk = nullptr;         // [npanels+1] first forward chunk of each box
+  uint2* fchunk = nullptr;             // [nfchunks+1] {first piece, first element}
+  uint32_t* brow = nullptr;            // [nseg] device row of each piece (backward order)
+  uint32_t* pchunk = nullptr;          // [npanels+1] first chunk of each box
+  uint2* chunk = nullptr;              // [nchunks+1] {first piece, first element}
+  uint16_t* boff = nullptr;            // [nchunks*(kBoxVox+1)] voxel element offsets
+  uint16_t* bpc = nullptr;             // [nnz] piece inside the chunk
+  float* bval = nullptr;               // [nnz]
+  uint32_t* colslot = nullptr;         // [cols+1] slot of each panel-order column (identity)
   double build_seconds = 0.0;
   PanelFormat() = default;
   PanelFormat(const PanelFormat&) = delete;
@@ -324,9 +330,14 @@ void launch_scale_rows(const DevMatrix& m, float* r, hipStream_t s);
 // tiling; free_panels releases it.
 const PanelFormat& ensure_panels(DevMatrix& m);
 void free_panels(DevMatrix& m);
-// part[fslot * W + f] = segment dot products with X (panel order, [cols][W]);
-// part[bslot * W + f] = group dot products with R ([rows][W], device rows).
-// grid: a multiple of 8 workgroups.
+// part[fslot * W + f] = piece dot products with X (panel order, [cols][W]);
+// part[c * W + f] = (A^T R)_c for every panel-order column c, R [rows][W] in
+// device row order (one slot per column: colslot is the identity).
+// One workgroup per box; grid is not used. For W <= 8 (panel_slots_by_box)
+// the forward writes piece s (forward order) to part[s * W + f] instead, so
+// that a box's sums are written contiguously; the slot sums of row r then
+// read part[rowpiece[k] * W + f] for k in [rowslot[r], rowslot[r+1]).
+constexpr bool panel_slots_by_box(int W) { return W <= 8; }
 template <int W>
 void launch_panel_fwd(const PanelFormat& P, const float* X, float* part, uint32_t grid,
                       hipStream_t s);

@@ -1,23 +1,24 @@
 // panel.hip — products of the system matrix with blocks of W interleaved
 // right-hand sides through its two-dimensional tiling (panels.hpp), gfx950.
 //
-// Forward, f = A X: a workgroup takes a tile of segments of one panel
-// ((row, panel) pieces of rows, at most kMaxSegment elements each), stages
-// their values and 16-bit in-panel columns in LDS with coalesced loads and
-// gives every segment to 8 lanes: W/4 quads of frames times 8/(W/4) shares of
-// the elements (every 8/(W/4)-th element). Per element a lane gathers one
-// 16-byte quad of X[(panel start + column) * W + ...] (the quads of the W/4
-// lanes make the element's W*4-byte run, from the panel's 2 MB slice of X at
-// W = 32) and accumulates with fused multiply-adds; a butterfly over the
-// lanes adds the shares and the segment's sums go to its slot. Backward,
-// u = A^T R: the same over the (panel, row chunk, voxel) groups, gathering
-// R[row * W + ...] from the chunk's rows. Workgroup b runs on XCD b % 8 and XCD x walks the
-// x-th contiguous eighth of the tiles, which are stored panel after panel
-// (chunk after chunk): the 256 workgroups of one XCD work through the same
-// panel (chunk) together, so its gathered slice stays in that XCD's L2.
+// Forward, f = A X: one 1024-lane workgroup per box loads the box's slice of
+// X (at most 1024 voxels x W floats, 128 KB at W = 32) into LDS with
+// coalesced 16-byte loads, then gives every piece of the box to 8 lanes: W/4
+// quads of frames times 8/(W/4) shares of the elements (every 8/(W/4)-th
+// element). Per element a lane reads its 16-byte quad of the voxel's run
+// from LDS and accumulates with fused multiply-adds; a butterfly over the
+// lanes adds the shares and the piece's sums go to its slot.
 //
-// The slots of one row (voxel) are contiguous; the epilogue kernels of the
-// callers add them in panel (chunk) order, so every result is deterministic.
+// Backward, u = A^T R: one 1024-lane workgroup per box; 8 lanes (quads times
+// shares again) own 8 voxels of the box, whose sums stay in registers while
+// the workgroup walks the box's pieces chunk by chunk: the chunk's runs of R
+// (at most 256 rows) and its elements (at most 4096, voxel by voxel) are
+// staged in LDS, then every lane group adds its voxels' elements. The
+// complete sums are written once per voxel.
+//
+// Boxes are launched largest first. The slots of one row are contiguous; the
+// epilogue kernels of the callers add them in box order, so every result is
+// deterministic.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -32,10 +33,7 @@
 namespace mlem {
 namespace {
 
-constexpr int kBlock = 256;
-constexpr uint32_t kPTile = 2048;  // elements staged per tile
-constexpr uint32_t kChunkRows = 16384;
-constexpr int kUnroll = 8;  // 16-byte gathers in flight per lane
+constexpr int kBoxThreads = 1024;
 
 void check_launch(const char* what) {
   hipError_t e = hipGetLastError();
@@ -55,35 +53,22 @@ void dfree(T*& p) {
   p = nullptr;
 }
 
-static_assert(kMaxSegment <= kPTile, "every segment fits a staged tile");
-
-// Greedy tiles over segments [s, e) with element offsets off[]: at most kBlock
-// segments and kPTile elements per tile (no segment is longer than that).
-void add_tiles(const std::vector<uint32_t>& off, uint32_t s, uint32_t e, uint32_t base,
-               std::vector<uint4>* tiles) {
-  while (s < e) {
-    uint32_t t = s + 1;
-    while (t < e && t - s < (uint32_t)kBlock && off[t + 1] - off[s] <= kPTile) ++t;
-    tiles->push_back(make_uint4(s, off[s], base, 0));
-    s = t;
-  }
-}
-
-// A lane carries four right-hand sides (one 16-byte gather per element): the
-// address unit handles a fixed number of lanes per clock, so 16-byte lanes
-// move four times the bytes of 4-byte ones. Every segment gets 8 lanes
-// whatever W is: W/4 lanes for the quads times E = 32/W lanes that take
-// every E-th element, their sums then added across the E lanes by a fixed
-// butterfly, so narrow blocks keep as many gathers in flight as wide ones.
+// Every piece (forward) and every voxel group (backward) gets 8 lanes
+// whatever W is: W/4 lanes for the quads of frames (16 bytes each) times
+// E = 32/W lanes that take every E-th element, their sums then added across
+// the E lanes by a fixed butterfly.
 template <int W>
 struct Quad {
   static_assert(W % 4 == 0 && W <= 32, "right-hand sides come in quads, at most 32");
-  static constexpr uint32_t L = W / 4;        // lanes per element (quads)
-  static constexpr uint32_t E = 8 / L;        // lanes per segment and quad
-  static constexpr uint32_t G = kBlock / 8;   // segments in flight per workgroup
+  static constexpr uint32_t L = W / 4;             // lanes per element (quads)
+  static constexpr uint32_t E = 8 / L;             // lanes per piece and quad
+  static constexpr uint32_t G = kBoxThreads / 8;   // pieces (voxel groups) per workgroup
 };
+// backward: voxels owned by one lane group (the groups cover a whole box)
+constexpr uint32_t kGroupVox = kBoxVox / (kBoxThreads / 8);
+static_assert(kGroupVox * (kBoxThreads / 8) == kBoxVox, "lane groups cover a box");
 
-// Sum over the E lanes (lane distance L) of a segment and quad; every one of
+// Sum over the E lanes (lane distance L) of a piece and quad; every one of
 // them ends with the same value (each step adds a pair in either order).
 template <int W>
 __device__ __forceinline__ float4 across_e(float4 v) {
@@ -98,29 +83,12 @@ __device__ __forceinline__ float4 across_e(float4 v) {
 }
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 // 16-byte store that does not allocate in the caches (partial sums are read
 // once, by the next kernel).
 __device__ __forceinline__ void store_nt(float4* p, const float4& v) {
   const f32x4 w = {v.x, v.y, v.z, v.w};
   __builtin_nontemporal_store(w, reinterpret_cast<f32x4*>(p));
-}
-
-__device__ __forceinline__ uint2 load_nt(const uint2* p) {
-  const u32x2 w = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p));
-  return make_uint2(w.x, w.y);
-}
-
-// Gathers through a buffer resource: one 32-bit offset register per load
-// instead of a 64-bit address (reads past `bytes` return zeros).
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* base, uint32_t bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
-                                           0x00020000);
-}
-__device__ __forceinline__ float4 gather4(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-  const f32x4 v = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
-  return make_float4(v.x, v.y, v.z, v.w);
 }
 
 __device__ __forceinline__ void fma4(float a, const float4& g, float4& acc) {
@@ -130,114 +98,246 @@ __device__ __forceinline__ void fma4(float a, const float4& g, float4& acc) {
   acc.w = fmaf(a, g.w, acc.w);
 }
 
-// This lane's share of one staged segment's dot products (elements b + ei,
-// b + ei + E, ... of [b, e)) with its four vectors: U gathers issued
-// together, then fused multiply-adds in element order. off0 is the byte
-// offset in X of the lane's quad of the panel's first column; each further
-// column is W*4 bytes on.
-template <int W, int U>
-__device__ __forceinline__ float4 pdot(const float* vals, const uint16_t* cols, uint32_t b,
-                                       uint32_t e, uint32_t ei, __amdgpu_buffer_rsrc_t rx,
-                                       uint32_t off0) {
-  constexpr uint32_t E = Quad<W>::E;
-  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (uint32_t k = b + ei; k < e; k += U * E) {
-    float4 g[U];
-#pragma unroll
-    for (int j = 0; j < U; ++j)
-      g[j] = gather4(rx, off0 + (uint32_t)cols[min(k + j * E, e - 1)] * (W * 4u));
-    // the values are read once the gathers are in flight (fewer live registers)
-#pragma unroll
-    for (int j = 0; j < U; ++j)
-      if (k + j * E < e) fma4(vals[k + j * E], g[j], acc);
+// Value of lane m of this lane's run of L consecutive lanes (L = 1, 2, 4, 8;
+// runs start at multiples of L), by DPP moves: quad permutes inside a quad,
+// plus a row shift by 4 lanes into the other quad of a run of 8.
+constexpr int qperm(int a, int b, int c, int d) { return a | (b << 2) | (c << 4) | (d << 6); }
+template <uint32_t L, uint32_t m>
+__device__ __forceinline__ uint32_t bcast(uint32_t x) {
+  static_assert(m < L && (L == 1 || L == 2 || L == 4 || L == 8), "runs of 1, 2, 4 or 8 lanes");
+  if constexpr (L == 1) {
+    return x;
+  } else if constexpr (L == 2) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, qperm(m, m, 2 + m, 2 + m), 0xF, 0xF, false);
+  } else if constexpr (L == 4) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, qperm(m, m, m, m), 0xF, 0xF, false);
+  } else {
+    constexpr int k = m % 4;
+    const int y = __builtin_amdgcn_mov_dpp((int)x, qperm(k, k, k, k), 0xF, 0xF, false);
+    if constexpr (m < 4)  // upper quads (banks 1, 3) take the lower quad's value: row_shr:4
+      return (uint32_t)__builtin_amdgcn_update_dpp(y, y, 0x114, 0xF, 0xA, false);
+    else  // lower quads (banks 0, 2) take the upper quad's value: row_shl:4
+      return (uint32_t)__builtin_amdgcn_update_dpp(y, y, 0x104, 0xF, 0x5, false);
   }
-  return across_e<W>(acc);
 }
 
-template <int W, int U>
-__device__ __forceinline__ float4 rdot(const uint2* ent, uint32_t b, uint32_t e, uint32_t ei,
-                                       __amdgpu_buffer_rsrc_t rr, uint32_t off0) {
-  constexpr uint32_t E = Quad<W>::E;
-  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (uint32_t k = b + ei; k < e; k += U * E) {
-    float4 g[U];
-#pragma unroll
-    for (int j = 0; j < U; ++j) g[j] = gather4(rr, off0 + ent[min(k + j * E, e - 1)].x * (W * 4u));
-#pragma unroll
-    for (int j = 0; j < U; ++j)
-      if (k + j * E < e) fma4(__uint_as_float(ent[k + j * E].y), g[j], acc);
+// Adds a batch of 8 staged elements [i, i + 8) of a list ending at e to the
+// lane's sums: lane g8 = ei*L + q of its run of 8 reads element i + g8 (past
+// e: weight 0) from LDS, and share ei takes elements i + ei*L + m, m < L,
+// from its run of L lanes; per element the lane reads its 16-byte quad of
+// the operand row idx from LDS. The reads go out in groups of up to four
+// before their multiply-adds.
+template <uint32_t L, uint32_t m, uint32_t n, uint32_t N>
+__device__ __forceinline__ void gather_rec(uint32_t idx, uint32_t vbits,
+                                           const float4* __restrict__ op, uint32_t q, float4* g,
+                                           float* v) {
+  if constexpr (n < N) {
+    g[n] = op[bcast<L, m + n>(idx) * L + q];
+    v[n] = __uint_as_float(bcast<L, m + n>(vbits));
+    gather_rec<L, m, n + 1, N>(idx, vbits, op, q, g, v);
   }
-  return across_e<W>(acc);
+}
+template <uint32_t L, uint32_t m = 0>
+__device__ __forceinline__ void batch8(uint32_t idx, float val, const float4* __restrict__ op,
+                                       uint32_t q, float4& acc) {
+  constexpr uint32_t N = L < 4 ? L : 4;
+  if constexpr (m < L) {
+    float4 g[N];
+    float v[N];
+    gather_rec<L, m, 0, N>(idx, __float_as_uint(val), op, q, g, v);
+#pragma unroll
+    for (uint32_t n = 0; n < N; ++n) fma4(v[n], g[n], acc);
+    batch8<L, m + N>(idx, val, op, q, acc);
+  }
 }
 
-// Forward tiles: part[fslot[s] * W + f] = sum over segment s of a * X[.].
-// Both tile kernels are held to 64 VGPRs (8 waves per SIMD): the launch is
-// a persistent grid of 8 workgroups per CU, all resident at once.
-template <int W>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_panel_fwd(
+// Staging: every prefetch load is unconditional (indices clamped into the
+// chunk) and every LDS store too (the arrays are padded to one entry per
+// thread; entries past the chunk hold copies that are never read), so the
+// loads are issued together, stay in flight while the current chunk is
+// reduced and are waited for only at the next chunk's stores.
+
+// Forward: part[fslot[s] * W + f] = sum over piece s of a * X[(c0 + col) * W + f]
+// for the pieces of box border[blockIdx.x]; X in panel order. The box's
+// pieces are staged in LDS chunk by chunk; the next chunk's elements and
+// piece offsets are loaded into registers while the current one is reduced.
+template <int W, bool kByBox>
+__global__ __launch_bounds__(kBoxThreads) void k_box_fwd(
+    const uint32_t* __restrict__ pstart, const uint32_t* __restrict__ border,
+    const uint32_t* __restrict__ pfchunk, const uint2* __restrict__ fchunk,
     const uint32_t* __restrict__ fseg, const uint32_t* __restrict__ fslot,
     const float* __restrict__ fval, const uint16_t* __restrict__ fcol,
-    const uint4* __restrict__ tiles, uint32_t ntiles, const float* __restrict__ X,
-    uint32_t xbytes, float* __restrict__ part) {
+    const float* __restrict__ X, float* __restrict__ part) {
   constexpr uint32_t L = Quad<W>::L, E = Quad<W>::E, G = Quad<W>::G;
-  __shared__ float sval[kPTile];
-  __shared__ uint16_t scol[kPTile];
-  __shared__ uint32_t S[kBlock + 1];
-  __shared__ uint32_t SL[kBlock];
-  const uint32_t t = threadIdx.x, q = t % L, ei = (t / L) % E, grp = t / 8;
-  const uint32_t xcd = blockIdx.x & 7u, per_xcd = gridDim.x >> 3;
-  const uint32_t t0 = (uint32_t)((uint64_t)ntiles * xcd / 8);
-  const uint32_t t1 = (uint32_t)((uint64_t)ntiles * (xcd + 1) / 8);
-  float4* __restrict__ part4 = reinterpret_cast<float4*>(part);
-  for (uint32_t tile = t0 + (blockIdx.x >> 3); tile < t1; tile += per_xcd) {
-    const uint4 d0 = tiles[tile], d1 = tiles[tile + 1];
-    const uint32_t s0 = d0.x, e0 = d0.y, nseg = d1.x - s0, count = d1.y - e0;
-    __syncthreads();  // the previous tile's readers are done with the LDS arrays
-    for (uint32_t i = t; i < count; i += kBlock) {  // streamed once: non-temporal
-      sval[i] = __builtin_nontemporal_load(fval + e0 + i);
-      scol[i] = __builtin_nontemporal_load(fcol + e0 + i);
+  constexpr uint32_t kPer = kFChunkElems / kBoxThreads;
+  constexpr uint32_t XJ = kBoxVox * L / kBoxThreads;
+  static_assert(kPer * kBoxThreads == kFChunkElems && kFChunkPieces < kBoxThreads, "staging");
+  __shared__ float4 xs[kBoxVox * L];
+  __shared__ float sval[kFChunkElems];
+  __shared__ uint16_t scol[kFChunkElems];
+  __shared__ uint16_t soff[kBoxThreads + 1];  // soff[s + 1]: end of piece s in the chunk
+  __shared__ uint32_t sslot[kBoxThreads];
+  const uint32_t box = border[blockIdx.x];
+  const uint32_t c0 = pstart[box], nv = pstart[box + 1] - c0;
+  const uint32_t t = threadIdx.x, q = t % L, ei = (t / L) % E, grp = t / 8, g8 = t % 8;
+  const uint32_t k0 = pfchunk[box], k1 = pfchunk[box + 1];
+  float rv[kPer];
+  uint32_t rc[kPer];
+  uint32_t rend = 0, rslot = 0;
+  auto fetch = [&](uint32_t k) __attribute__((always_inline)) {  // chunk k into registers
+    const uint2 a = fchunk[k], b = fchunk[k + 1];
+    const uint32_t ne = b.y - a.y, np = b.x - a.x;
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {  // streamed once: non-temporal
+      const uint32_t i = a.y + min(t + j * kBoxThreads, ne - 1);
+      rv[j] = __builtin_nontemporal_load(fval + i);
+      rc[j] = __builtin_nontemporal_load(fcol + i);
     }
-    for (uint32_t i = t; i <= nseg; i += kBlock) S[i] = fseg[s0 + i] - e0;
-    for (uint32_t i = t; i < nseg; i += kBlock) SL[i] = fslot[s0 + i];
+    const uint32_t sp = a.x + min(t, np - 1);
+    rend = fseg[sp + 1];
+    if constexpr (!kByBox) rslot = fslot[sp];
+  };
+  if (k0 < k1) fetch(k0);
+  {  // the box's slice of X
+    const float4* __restrict__ X4 = reinterpret_cast<const float4*>(X) + (size_t)c0 * L;
+    const uint32_t n = nv * L;
+    float4 xr[XJ];
+#pragma unroll
+    for (uint32_t j = 0; j < XJ; ++j) xr[j] = X4[min(t + j * kBoxThreads, n - 1)];
+#pragma unroll
+    for (uint32_t j = 0; j < XJ; ++j) xs[t + j * kBoxThreads] = xr[j];
+  }
+  if (t == 0) soff[0] = 0;
+  float4* __restrict__ part4 = reinterpret_cast<float4*>(part);
+  for (uint32_t k = k0; k < k1; ++k) {
+    const uint2 a = fchunk[k], b = fchunk[k + 1];
+    const uint32_t np = b.x - a.x;
+    __syncthreads();  // the previous chunk's readers are done with the LDS arrays
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {
+      sval[t + j * kBoxThreads] = rv[j];
+      scol[t + j * kBoxThreads] = (uint16_t)rc[j];
+    }
+    soff[t + 1] = (uint16_t)(rend - a.y);
+    sslot[t] = kByBox ? a.x + t : rslot;
     __syncthreads();
-    const auto rx = rsrc(X, xbytes);
-    const uint32_t off0 = d0.z * (W * 4u) + q * 16u;
-    for (uint32_t sg = grp; sg < nseg; sg += G) {
-      const float4 d = pdot<W, kUnroll>(sval, scol, S[sg], S[sg + 1], ei, rx, off0);
-      if (ei == 0) store_nt(part4 + (size_t)SL[sg] * L + q, d);
+    if (k + 1 < k1) fetch(k + 1);
+    for (uint32_t sg = grp; sg < np; sg += G) {
+      const uint32_t e = soff[sg + 1];
+      float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+      for (uint32_t i = soff[sg]; i < e; i += 8) {
+        const uint32_t my = i + g8, ic = min(my, e - 1);
+        const uint32_t c = scol[ic];
+        const float v = sval[ic];
+        batch8<L>(c, my < e ? v : 0.f, xs, q, acc);
+      }
+      acc = across_e<W>(acc);
+      if (ei == 0) store_nt(part4 + (size_t)sslot[sg] * L + q, acc);
     }
   }
 }
 
-// Backward tiles: part[bslot[g] * W + f] = sum over group g of a * R[row * W + f].
+// Backward: U[(c0 + v) * W + f] = sum over the box's elements of voxel v of
+// a * R[row * W + f] (R in device row order), for box border[blockIdx.x].
+// Lane group g owns voxels 8g .. 8g+7; per chunk the group adds those
+// voxels' staged elements in order (E shares of every voxel's list). While a
+// chunk is reduced, the next chunk's runs of R, elements and voxel offsets
+// are loaded into registers, and the rows of the one after it.
 template <int W>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_panel_bwd(
-    const uint32_t* __restrict__ bseg, const uint32_t* __restrict__ bslot,
-    const uint2* __restrict__ bent, const uint4* __restrict__ tiles, uint32_t ntiles,
-    const float* __restrict__ R, uint32_t rbytes, float* __restrict__ part) {
-  constexpr uint32_t L = Quad<W>::L, E = Quad<W>::E, G = Quad<W>::G;
-  __shared__ uint2 sent[kPTile];
-  __shared__ uint32_t S[kBlock + 1];
-  __shared__ uint32_t SL[kBlock];
-  const uint32_t t = threadIdx.x, q = t % L, ei = (t / L) % E, grp = t / 8;
-  const uint32_t xcd = blockIdx.x & 7u, per_xcd = gridDim.x >> 3;
-  const uint32_t t0 = (uint32_t)((uint64_t)ntiles * xcd / 8);
-  const uint32_t t1 = (uint32_t)((uint64_t)ntiles * (xcd + 1) / 8);
-  const auto rr = rsrc(R, rbytes);
-  const uint32_t off0 = q * 16u;
-  float4* __restrict__ part4 = reinterpret_cast<float4*>(part);
-  for (uint32_t tile = t0 + (blockIdx.x >> 3); tile < t1; tile += per_xcd) {
-    const uint4 d0 = tiles[tile], d1 = tiles[tile + 1];
-    const uint32_t s0 = d0.x, e0 = d0.y, nseg = d1.x - s0, count = d1.y - e0;
-    __syncthreads();
-    for (uint32_t i = t; i < count; i += kBlock) sent[i] = load_nt(bent + e0 + i);
-    for (uint32_t i = t; i <= nseg; i += kBlock) S[i] = bseg[s0 + i] - e0;
-    for (uint32_t i = t; i < nseg; i += kBlock) SL[i] = bslot[s0 + i];
-    __syncthreads();
-    for (uint32_t sg = grp; sg < nseg; sg += G) {
-      const float4 d = rdot<W, kUnroll>(sent, S[sg], S[sg + 1], ei, rr, off0);
-      if (ei == 0) store_nt(part4 + (size_t)SL[sg] * L + q, d);
+__global__ __launch_bounds__(kBoxThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_box_bwd(
+    const uint32_t* __restrict__ pstart, const uint32_t* __restrict__ border,
+    const uint32_t* __restrict__ pchunk, const uint2* __restrict__ chunk,
+    const uint16_t* __restrict__ boff, const uint32_t* __restrict__ brow,
+    const uint16_t* __restrict__ bpc, const float* __restrict__ bval,
+    const float* __restrict__ R, float* __restrict__ U) {
+  constexpr uint32_t L = Quad<W>::L, E = Quad<W>::E, V = kGroupVox;
+  constexpr uint32_t kPer = kChunkElems / kBoxThreads;
+  constexpr uint32_t kRq = (kChunkPieces * L + kBoxThreads - 1) / kBoxThreads;  // R quads
+  static_assert(kPer * kBoxThreads == kChunkElems && kBoxVox == kBoxThreads, "staging");
+  __shared__ float4 rs[kRq * kBoxThreads];
+  __shared__ float sval[kChunkElems];
+  __shared__ uint16_t spc[kChunkElems];
+  __shared__ uint16_t soff[kBoxVox + 1];  // soff[v + 1]: end of voxel v's elements
+  const uint32_t box = border[blockIdx.x];
+  const uint32_t c0 = pstart[box], nv = pstart[box + 1] - c0;
+  const uint32_t t = threadIdx.x, q = t % L, ei = (t / L) % E, v0 = (t / 8) * V, g8 = t % 8;
+  const uint32_t k0 = pchunk[box], k1 = pchunk[box + 1];
+  const float4* __restrict__ R4 = reinterpret_cast<const float4*>(R);
+  // vector registers (not arrays), so they are never placed in memory
+  typedef float rq_t __attribute__((ext_vector_type(4 * kRq)));
+  typedef uint32_t rr_t __attribute__((ext_vector_type(kRq)));
+  rq_t rR;
+  rr_t rrow;
+  float rv[kPer];
+  uint32_t rp[kPer];
+  uint32_t rend = 0;
+  auto rows_of = [&](uint32_t k) __attribute__((always_inline)) {  // rows of chunk k's R quads
+    const uint2 a = chunk[k], b = chunk[k + 1];
+    const uint32_t nq = (b.x - a.x) * L;
+#pragma unroll
+    for (uint32_t j = 0; j < kRq; ++j) rrow[j] = brow[a.x + min(t + j * kBoxThreads, nq - 1) / L];
+
+  };
+  auto fetch = [&](uint32_t k) __attribute__((always_inline)) {  // chunk k (its rows in rrow)
+    const uint2 a = chunk[k], b = chunk[k + 1];
+    const uint32_t ne = b.y - a.y;
+#pragma unroll
+    for (uint32_t j = 0; j < kRq; ++j) {
+      const float4 g = R4[(size_t)rrow[j] * L + (t + j * kBoxThreads) % L];
+      rR[4 * j] = g.x;
+      rR[4 * j + 1] = g.y;
+      rR[4 * j + 2] = g.z;
+      rR[4 * j + 3] = g.w;
     }
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {  // streamed once: non-temporal
+      const uint32_t i = a.y + min(t + j * kBoxThreads, ne - 1);
+      rv[j] = __builtin_nontemporal_load(bval + i);
+      rp[j] = __builtin_nontemporal_load(bpc + i);
+    }
+    rend = boff[(size_t)k * (kBoxVox + 1) + t + 1];
+  };
+  if (k0 < k1) {
+    rows_of(k0);
+    fetch(k0);
+    if (k0 + 1 < k1) rows_of(k0 + 1);
+  }
+  if (t == 0) soff[0] = 0;
+  float4 acc[V];
+#pragma unroll
+  for (uint32_t j = 0; j < V; ++j) acc[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (uint32_t k = k0; k < k1; ++k) {
+    __syncthreads();  // the previous chunk's readers are done with the LDS arrays
+#pragma unroll
+    for (uint32_t j = 0; j < kRq; ++j)
+      rs[t + j * kBoxThreads] = make_float4(rR[4 * j], rR[4 * j + 1], rR[4 * j + 2], rR[4 * j + 3]);
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {
+      sval[t + j * kBoxThreads] = rv[j];
+      spc[t + j * kBoxThreads] = (uint16_t)rp[j];
+    }
+    soff[t + 1] = (uint16_t)rend;
+    __syncthreads();
+    if (k + 1 < k1) {
+      fetch(k + 1);
+      if (k + 2 < k1) rows_of(k + 2);
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < V; ++j) {
+      const uint32_t e = soff[v0 + j + 1];
+      for (uint32_t i = soff[v0 + j]; i < e; i += 8) {
+        const uint32_t my = i + g8, ic = min(my, e - 1);
+        const uint32_t pc = spc[ic];
+        const float v = sval[ic];
+        batch8<L>(pc, my < e ? v : 0.f, rs, q, acc[j]);
+      }
+    }
+  }
+  float4* __restrict__ U4 = reinterpret_cast<float4*>(U);
+#pragma unroll
+  for (uint32_t j = 0; j < V; ++j) {
+    const float4 u = across_e<W>(acc[j]);
+    if (ei == 0 && v0 + j < nv) U4[(size_t)(c0 + v0 + j) * L + q] = u;
   }
 }
 
@@ -248,17 +348,24 @@ PanelFormat::~PanelFormat() {
   dfree(pc_of_dev);
   dfree(dev_of_pc);
   dfree(norm);
+  dfree(pstart);
+  dfree(border);
+  dfree(pseg);
   dfree(fseg);
   dfree(fslot);
   dfree(fval);
   dfree(fcol);
   dfree(rowslot);
-  dfree(ftiles);
-  dfree(bseg);
-  dfree(bslot);
-  dfree(bent);
+  dfree(rowpiece);
+  dfree(pfchunk);
+  dfree(fchunk);
+  dfree(brow);
+  dfree(pchunk);
+  dfree(chunk);
+  dfree(boff);
+  dfree(bpc);
+  dfree(bval);
   dfree(colslot);
-  dfree(btiles);
 }
 
 const PanelFormat& ensure_panels(DevMatrix& m) {
@@ -281,8 +388,9 @@ const PanelFormat& ensure_panels(DevMatrix& m) {
   if (!m.colmap) std::iota(colmap.begin(), colmap.end(), 0u);
 
   std::vector<uint32_t> pc_of_file, pstart;
+  uint32_t boxgrid[3] = {0, 0, 0};
   const bool has_grid = (uint64_t)m.grid[0] * m.grid[1] * m.grid[2] == m.cols && m.cols;
-  panel_order(m.cols, has_grid ? m.grid : nullptr, &pc_of_file, &pstart);
+  panel_order(m.cols, has_grid ? m.grid : nullptr, &pc_of_file, &pstart, boxgrid);
   std::vector<uint32_t> pc_of_dev(m.cols), dev_of_pc(m.cols);
   for (uint32_t c = 0; c < m.cols; ++c) pc_of_dev[colmap[c]] = pc_of_file[c];
   for (uint32_t d = 0; d < m.cols; ++d) dev_of_pc[pc_of_dev[d]] = d;
@@ -292,48 +400,53 @@ const PanelFormat& ensure_panels(DevMatrix& m) {
   HostPanels H;
   try {
     build_host_panels(m.rows, m.cols, m.nnz, rowptr.data(), csr2.data(), pc_of_dev, pstart,
-                      kChunkRows, 0, &H);
+                      boxgrid, 0, &H);
   } catch (const std::invalid_argument& e) {
     raise(MLEM_EINVAL, std::string("matrix tiling: ") + e.what());
   }
   std::vector<uint32_t>().swap(csr2);
-  std::vector<uint4> ft, bt;
-  for (uint32_t p = 0; p < H.npanels; ++p) add_tiles(H.fseg, H.pseg[p], H.pseg[p + 1], H.pstart[p], &ft);
-  ft.push_back(make_uint4((uint32_t)(H.fseg.size() - 1), (uint32_t)m.nnz, 0, 0));
-  const uint32_t nchunks = (uint32_t)H.cgroup.size() - 1;
-  for (uint32_t c = 0; c < nchunks; ++c) add_tiles(H.bseg, H.cgroup[c], H.cgroup[c + 1], 0, &bt);
-  bt.push_back(make_uint4((uint32_t)(H.bseg.size() - 1), (uint32_t)m.nnz, 0, 0));
-  std::vector<uint2> bent(m.nnz);
-  for (uint64_t k = 0; k < m.nnz; ++k) {
-    uint32_t bits;
-    std::memcpy(&bits, &H.bval[k], 4);
-    bent[k] = make_uint2(H.brow[k], bits);
-  }
+  const uint32_t nchunks = (uint32_t)H.chunk_piece.size() - 1;
+  const uint32_t nfchunks = (uint32_t)H.fchunk_piece.size() - 1;
+  std::vector<uint2> fchunk((size_t)nfchunks + 1);
+  for (uint32_t k = 0; k <= nfchunks; ++k)
+    fchunk[k] = make_uint2(H.fchunk_piece[k], H.fseg[H.fchunk_piece[k]]);
+  std::vector<uint2> chunk((size_t)nchunks + 1);
+  for (uint32_t k = 0; k <= nchunks; ++k) chunk[k] = make_uint2(H.chunk_piece[k], H.chunk_elem[k]);
+  std::vector<uint32_t> colslot((size_t)m.cols + 1);
+  std::iota(colslot.begin(), colslot.end(), 0u);
 
   std::unique_ptr<PanelFormat> P(new PanelFormat());
   P->rows = m.rows;
   P->cols = m.cols;
   P->npanels = H.npanels;
   P->nchunks = nchunks;
+  P->nfchunks = nfchunks;
   P->nseg = H.fseg.size() - 1;
-  P->ngroups = H.bseg.size() - 1;
-  P->nftiles = (uint32_t)ft.size() - 1;
-  P->nbtiles = (uint32_t)bt.size() - 1;
+  P->ngroups = m.cols;
   upload(&P->pc_of_file, pc_of_file, s);
   upload(&P->pc_of_dev, pc_of_dev, s);
   upload(&P->dev_of_pc, dev_of_pc, s);
   upload(&P->norm, pnorm, s);
+  upload(&P->pstart, H.pstart, s);
+  upload(&P->border, H.border, s);
+  upload(&P->pseg, H.pseg, s);
   upload(&P->fseg, H.fseg, s);
   upload(&P->fslot, H.fslot, s);
   upload(&P->fval, H.fval, s);
   upload(&P->fcol, H.fcol, s);
   upload(&P->rowslot, H.rowslot, s);
-  upload(&P->ftiles, ft, s);
-  upload(&P->bseg, H.bseg, s);
-  upload(&P->bslot, H.bslot, s);
-  upload(&P->bent, bent, s);
-  upload(&P->colslot, H.colslot, s);
-  upload(&P->btiles, bt, s);
+  std::vector<uint32_t> rowpiece(H.fslot.size());
+  for (size_t k = 0; k < H.fslot.size(); ++k) rowpiece[H.fslot[k]] = (uint32_t)k;
+  upload(&P->rowpiece, rowpiece, s);
+  upload(&P->pfchunk, H.pfchunk, s);
+  upload(&P->fchunk, fchunk, s);
+  upload(&P->brow, H.brow, s);
+  upload(&P->pchunk, H.pchunk, s);
+  upload(&P->chunk, chunk, s);
+  upload(&P->boff, H.boff, s);
+  upload(&P->bpc, H.bpc, s);
+  upload(&P->bval, H.bval, s);
+  upload(&P->colslot, colslot, s);
   MLEM_HIP(hipStreamSynchronize(s));  // host vectors go out of scope
   P->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   m.panels = P.release();
@@ -345,26 +458,21 @@ void free_panels(DevMatrix& m) {
   m.panels = nullptr;
 }
 
-// Byte size of a [n][W] fp32 block for a buffer resource (32-bit sizes).
-uint32_t block_bytes(uint64_t n, int W) {
-  const uint64_t b = 4ull * n * (uint64_t)W;
-  if (b > 0xFFFFFFFFull) raise(MLEM_EINVAL, "right-hand-side block larger than 4 GiB");
-  return (uint32_t)b;
-}
-
 template <int W>
-void launch_panel_fwd(const PanelFormat& P, const float* X, float* part, uint32_t grid,
+void launch_panel_fwd(const PanelFormat& P, const float* X, float* part, uint32_t /*grid*/,
                       hipStream_t s) {
-  hipLaunchKernelGGL((k_panel_fwd<W>), dim3(grid), dim3(kBlock), 0, s, P.fseg, P.fslot, P.fval,
-                     P.fcol, P.ftiles, P.nftiles, X, block_bytes(P.cols, W), part);
+  if (!P.npanels) return;
+  hipLaunchKernelGGL((k_box_fwd<W, panel_slots_by_box(W)>), dim3(P.npanels), dim3(kBoxThreads), 0, s, P.pstart,
+                     P.border, P.pfchunk, P.fchunk, P.fseg, P.fslot, P.fval, P.fcol, X, part);
   check_launch("tiled forward launch");
 }
 
 template <int W>
-void launch_panel_bwd(const PanelFormat& P, const float* R, float* part, uint32_t grid,
+void launch_panel_bwd(const PanelFormat& P, const float* R, float* part, uint32_t /*grid*/,
                       hipStream_t s) {
-  hipLaunchKernelGGL((k_panel_bwd<W>), dim3(grid), dim3(kBlock), 0, s, P.bseg, P.bslot, P.bent,
-                     P.btiles, P.nbtiles, R, block_bytes(P.rows, W), part);
+  if (!P.npanels) return;
+  hipLaunchKernelGGL((k_box_bwd<W>), dim3(P.npanels), dim3(kBoxThreads), 0, s, P.pstart,
+                     P.border, P.pchunk, P.chunk, P.boff, P.brow, P.bpc, P.bval, R, part);
   check_launch("tiled backward launch");
 }
 

@@ -10,8 +10,11 @@
 namespace mlem {
 namespace {
 
-constexpr uint32_t kPanelX = 32, kPanelY = 32, kPanelZ = 16;  // 16384 voxels
-constexpr uint32_t kFlatPanel = 16384;                          // without a grid
+constexpr uint32_t kBoxX = 16, kBoxY = 16, kBoxZ = 4;
+static_assert(kBoxX * kBoxY * kBoxZ == kBoxVox, "box shape");
+static_assert(kMaxSegment <= kChunkElems && kMaxSegment <= kFChunkElems,
+              "every piece fits a chunk");
+static_assert(kChunkPieces <= 65536 && kChunkElems <= 65535, "16-bit chunk offsets");
 
 int pick_threads(int threads) {
   if (threads > 0) return threads;
@@ -34,28 +37,30 @@ void parallel_for(int nthreads, F&& f) {
 }  // namespace
 
 void panel_order(uint32_t cols, const uint32_t grid[3], std::vector<uint32_t>* pc_of_file,
-                 std::vector<uint32_t>* pstart) {
+                 std::vector<uint32_t>* pstart, uint32_t boxgrid[3]) {
   pc_of_file->assign(cols, 0);
   pstart->clear();
   const uint64_t nx = grid ? grid[0] : 0, ny = grid ? grid[1] : 0, nz = grid ? grid[2] : 0;
   if (!nx || !ny || !nz || nx * ny * nz != cols) {
     for (uint32_t c = 0; c < cols; ++c) (*pc_of_file)[c] = c;
-    for (uint64_t p = 0; p < cols; p += kFlatPanel) pstart->push_back((uint32_t)p);
+    for (uint64_t p = 0; p < cols; p += kBoxVox) pstart->push_back((uint32_t)p);
     pstart->push_back(cols);
+    boxgrid[0] = (uint32_t)pstart->size() - 1;
+    boxgrid[1] = boxgrid[2] = 1;
     return;
   }
-  const uint64_t npx = (nx + kPanelX - 1) / kPanelX, npy = (ny + kPanelY - 1) / kPanelY;
-  const uint64_t npz = (nz + kPanelZ - 1) / kPanelZ;
-  // sort key: panel, then 4x4x2 brick inside the panel (x fastest), then the
-  // voxel inside the brick; ranks of the keys are the panel-order columns
+  const uint64_t nbx = (nx + kBoxX - 1) / kBoxX, nby = (ny + kBoxY - 1) / kBoxY;
+  boxgrid[0] = (uint32_t)nbx;
+  boxgrid[1] = (uint32_t)nby;
+  boxgrid[2] = (uint32_t)((nz + kBoxZ - 1) / kBoxZ);
+  // sort key: box (x fastest), then the voxel inside the box (x fastest);
+  // ranks of the keys are the panel-order columns
   std::vector<uint64_t> key(cols);
   for (uint64_t c = 0; c < cols; ++c) {
     const uint64_t x = c % nx, y = (c / nx) % ny, z = c / (nx * ny);
-    const uint64_t p = x / kPanelX + npx * (y / kPanelY + npy * (z / kPanelZ));
-    const uint64_t lx = x % kPanelX, ly = y % kPanelY, lz = z % kPanelZ;
-    const uint64_t b = lx / 4 + (kPanelX / 4) * (ly / 4 + (kPanelY / 4) * (lz / 2));
-    const uint64_t in = lx % 4 + 4 * (ly % 4 + 4 * (lz % 2));
-    key[c] = (p << 32) | (b * 32 + in);
+    const uint64_t p = x / kBoxX + nbx * (y / kBoxY + nby * (z / kBoxZ));
+    const uint64_t in = x % kBoxX + kBoxX * (y % kBoxY + kBoxY * (z % kBoxZ));
+    key[c] = (p << 32) | in;
   }
   std::vector<uint32_t> order(cols);
   std::iota(order.begin(), order.end(), 0u);
@@ -71,30 +76,28 @@ void panel_order(uint32_t cols, const uint32_t grid[3], std::vector<uint32_t>* p
     }
   }
   pstart->push_back(cols);
-  (void)npz;
 }
 
 void build_host_panels(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t* rowptr,
                        const uint32_t* csr2, const std::vector<uint32_t>& pc_of_dev,
-                       const std::vector<uint32_t>& pstart, uint32_t chunk_rows, int threads,
-                       HostPanels* out) {
+                       const std::vector<uint32_t>& pstart, const uint32_t boxgrid[3],
+                       int threads, HostPanels* out) {
   HostPanels& H = *out;
   H = HostPanels();
   H.rows = rows;
   H.cols = cols;
   H.nnz = nnz;
-  H.chunk_rows = chunk_rows;
   H.pc_of_dev = pc_of_dev;
   H.pstart = pstart;
   const uint32_t P = (uint32_t)pstart.size() - 1;
   H.npanels = P;
   if (pc_of_dev.size() != cols) throw std::invalid_argument("panel order size");
-  if (chunk_rows == 0) throw std::invalid_argument("chunk size");
+  if ((uint64_t)boxgrid[0] * boxgrid[1] * boxgrid[2] != P) throw std::invalid_argument("box grid");
   if (nnz > 0xFFFFFFFFull) throw std::invalid_argument("too many elements for 32-bit offsets");
   for (uint32_t p = 0; p < P; ++p)
-    if (pstart[p + 1] - pstart[p] > 65536u) throw std::invalid_argument("panel too large");
+    if (pstart[p + 1] - pstart[p] > kBoxVox) throw std::invalid_argument("box too large");
 
-  // panel and in-panel offset of every device column
+  // box and in-box offset of every device column
   std::vector<uint32_t> pid(cols);
   std::vector<uint16_t> lid(cols);
   {
@@ -103,6 +106,7 @@ void build_host_panels(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_
       for (uint32_t c = pstart[p]; c < pstart[p + 1]; ++c) panel_of_pc[c] = p;
     for (uint32_t d = 0; d < cols; ++d) {
       const uint32_t pc = pc_of_dev[d];
+      if (pc >= cols) throw std::invalid_argument("panel order out of range");
       pid[d] = panel_of_pc[pc];
       lid[d] = (uint16_t)(pc - pstart[pid[d]]);
     }
@@ -118,7 +122,7 @@ void build_host_panels(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_
   }
   for (int t = 1; t <= T; ++t) rr[(size_t)t] = std::max(rr[(size_t)t], rr[(size_t)t - 1]);
 
-  // ---- forward, pass 1: segments per row, segments and elements per (thread, panel)
+  // ---- pass 1: pieces per row, pieces and elements per (thread, box)
   std::vector<uint32_t> nsegrow(rows);
   std::vector<std::vector<uint64_t>> cseg((size_t)T, std::vector<uint64_t>(P, 0));
   std::vector<std::vector<uint64_t>> cel((size_t)T, std::vector<uint64_t>(P, 0));
@@ -139,7 +143,7 @@ void build_host_panels(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_
         cnt[p]++;
       }
       uint32_t nd = 0;
-      for (uint32_t p : plist) {  // a (row, panel) piece longer than SP is split
+      for (uint32_t p : plist) {  // a (row, box) piece longer than SP is split
         const uint32_t pieces = (cnt[p] + SP - 1) / SP;
         nd += pieces;
         cseg[(size_t)t][p] += pieces;
@@ -150,8 +154,8 @@ void build_host_panels(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_
   H.rowslot.assign((size_t)rows + 1, 0);
   for (uint32_t r = 0; r < rows; ++r) H.rowslot[r + 1] = H.rowslot[r] + nsegrow[r];
   const uint64_t nseg = H.rowslot[rows];
-  if (nseg > 0xFFFFFFF0ull) throw std::invalid_argument("too many segments");
-  // panel-major cursors: panel p, then the threads in row order
+  if (nseg > 0xFFFFFFF0ull) throw std::invalid_argument("too many pieces");
+  // box-major cursors: box p, then the threads in row order
   std::vector<std::vector<uint64_t>> scur((size_t)T, std::vector<uint64_t>(P)),
       ecur((size_t)T, std::vector<uint64_t>(P));
   H.pseg.assign((size_t)P + 1, 0);
@@ -171,12 +175,12 @@ void build_host_panels(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_
     H.pseg[P] = (uint32_t)s;
     pel[P] = e;
   }
-  // ---- forward, pass 2: segment offsets and slots, elements
-  H.fseg.assign(nseg + 1, 0);
-  H.fslot.assign(nseg, 0);
-  H.fval.assign(nnz, 0.f);
-  H.fcol.assign(nnz, 0);
-  std::vector<uint32_t> segrow(nseg);
+  // ---- pass 2: pieces in row order inside each box (the backward order):
+  // first element, slot and row of every piece, elements piece-major
+  std::vector<uint32_t> rseg(nseg + 1), rslot(nseg);
+  std::vector<float> rval(nnz);
+  std::vector<uint16_t> rcol(nnz);
+  H.brow.assign(nseg, 0);
   parallel_for(T, [&](int t) {
     std::vector<uint32_t> stamp(P, 0xFFFFFFFFu), cnt(P, 0);
     std::vector<uint64_t> pos(P, 0);
@@ -199,108 +203,127 @@ void build_host_panels(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_
         pos[p] = ecur[(size_t)t][p];
         for (uint32_t done = 0; done < cnt[p]; done += SP) {
           const uint64_t s = scur[(size_t)t][p]++;
-          H.fseg[s] = (uint32_t)(ecur[(size_t)t][p] + done);
-          H.fslot[s] = H.rowslot[r] + rank++;
-          segrow[s] = r;
+          rseg[s] = (uint32_t)(ecur[(size_t)t][p] + done);
+          rslot[s] = H.rowslot[r] + rank++;
+          H.brow[s] = r;
         }
         ecur[(size_t)t][p] += cnt[p];
       }
       for (uint32_t k = rowptr[r]; k < rowptr[r + 1]; ++k) {
         const uint32_t d = csr2[2ull * k];
         const uint64_t q = pos[pid[d]]++;
-        uint32_t bits = csr2[2ull * k + 1];
+        const uint32_t bits = csr2[2ull * k + 1];
         float v;
         static_assert(sizeof(v) == sizeof(bits), "fp32");
         std::memcpy(&v, &bits, 4);
-        H.fval[q] = v;
-        H.fcol[q] = lid[d];
+        rval[q] = v;
+        rcol[q] = lid[d];
+      }
+    }
+  });
+  rseg[nseg] = (uint32_t)nnz;
+
+  const int TP = std::max(1, std::min<int>(T, (int)std::max<uint32_t>(P, 1)));
+  // ---- forward: inside every box the pieces longest first (stable)
+  H.fseg.assign(nseg + 1, 0);
+  H.fslot.assign(nseg, 0);
+  H.fval.assign(nnz, 0.f);
+  H.fcol.assign(nnz, 0);
+  parallel_for(TP, [&](int t) {
+    std::vector<uint32_t> idx;
+    for (uint32_t p = (uint32_t)t; p < P; p += (uint32_t)TP) {
+      const uint32_t s0 = H.pseg[p], s1 = H.pseg[p + 1];
+      idx.resize(s1 - s0);
+      std::iota(idx.begin(), idx.end(), s0);
+      std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
+        return rseg[a + 1] - rseg[a] > rseg[b + 1] - rseg[b];
+      });
+      uint64_t e = pel[p];
+      for (uint32_t i = 0; i < s1 - s0; ++i) {
+        const uint32_t s = idx[i], n = rseg[s + 1] - rseg[s];
+        H.fseg[s0 + i] = (uint32_t)e;
+        H.fslot[s0 + i] = rslot[s];
+        std::memcpy(&H.fval[e], &rval[rseg[s]], 4ull * n);
+        std::memcpy(&H.fcol[e], &rcol[rseg[s]], 2ull * n);
+        e += n;
       }
     }
   });
   H.fseg[nseg] = (uint32_t)nnz;
-
-  // ---- backward: groups (panel, chunk, voxel)
-  const uint32_t C = chunk_rows;
-  H.pchunk.assign((size_t)P + 1, 0);
+  // forward chunks: greedy over the length-sorted pieces of every box
+  auto fchunk_end = [&](uint32_t s, uint32_t s1) {
+    uint32_t e = s + 1;
+    while (e < s1 && e - s < kFChunkPieces && H.fseg[e + 1] - H.fseg[s] <= kFChunkElems) ++e;
+    return e;
+  };
+  H.pfchunk.assign((size_t)P + 1, 0);
   for (uint32_t p = 0; p < P; ++p) {
-    const uint32_t ns = H.pseg[p + 1] - H.pseg[p];
-    H.pchunk[p + 1] = H.pchunk[p] + (ns + C - 1) / C;
+    uint32_t n = 0;
+    for (uint32_t s = H.pseg[p]; s < H.pseg[p + 1]; s = fchunk_end(s, H.pseg[p + 1])) ++n;
+    H.pfchunk[p + 1] = H.pfchunk[p] + n;
   }
+  H.fchunk_piece.assign((size_t)H.pfchunk[P] + 1, 0);
+  for (uint32_t p = 0; p < P; ++p) {
+    uint32_t k = H.pfchunk[p];
+    for (uint32_t s = H.pseg[p]; s < H.pseg[p + 1]; s = fchunk_end(s, H.pseg[p + 1]))
+      H.fchunk_piece[k++] = s;
+  }
+  H.fchunk_piece[H.pfchunk[P]] = (uint32_t)nseg;
+
+  // ---- backward: chunks of the row-ordered pieces of every box
+  std::vector<uint32_t> nck(P, 0);
+  auto chunk_end = [&](uint32_t s, uint32_t s1) {  // greedy: pieces and elements bounded
+    uint32_t e = s + 1;
+    while (e < s1 && e - s < kChunkPieces && rseg[e + 1] - rseg[s] <= kChunkElems) ++e;
+    return e;
+  };
+  parallel_for(TP, [&](int t) {
+    for (uint32_t p = (uint32_t)t; p < P; p += (uint32_t)TP)
+      for (uint32_t s = H.pseg[p]; s < H.pseg[p + 1]; s = chunk_end(s, H.pseg[p + 1])) nck[p]++;
+  });
+  H.pchunk.assign((size_t)P + 1, 0);
+  for (uint32_t p = 0; p < P; ++p) H.pchunk[p + 1] = H.pchunk[p] + nck[p];
   const uint32_t nchunks = H.pchunk[P];
-  // phase A (threads over panels): entries per group; non-empty chunks per voxel
-  std::vector<std::vector<uint32_t>> gcount(P);
-  std::vector<uint32_t> nchunk_pc(cols, 0);
-  std::vector<uint64_t> ngroups_p(P, 0);
-  const int TP = std::max(1, std::min<int>(T, (int)std::max<uint32_t>(P, 1)));
-  parallel_for(TP, [&](int t) {
-    for (uint32_t p = (uint32_t)t; p < P; p += (uint32_t)TP) {
-      const uint32_t pv = pstart[p + 1] - pstart[p];
-      const uint32_t nk = H.pchunk[p + 1] - H.pchunk[p];
-      std::vector<uint32_t>& g = gcount[p];
-      g.assign((size_t)nk * pv, 0);
-      for (uint32_t s = H.pseg[p]; s < H.pseg[p + 1]; ++s) {
-        const uint32_t k = (s - H.pseg[p]) / C;
-        for (uint32_t e = H.fseg[s]; e < H.fseg[s + 1]; ++e) g[(size_t)k * pv + H.fcol[e]]++;
-      }
-      uint64_t ng = 0;
-      for (uint32_t v = 0; v < pv; ++v) {
-        uint32_t n = 0;  // groups of the voxel: non-empty chunks, long ones split
-        for (uint32_t k = 0; k < nk; ++k) n += (g[(size_t)k * pv + v] + SP - 1) / SP;
-        nchunk_pc[pstart[p] + v] = n;
-        ng += n;
-      }
-      ngroups_p[p] = ng;
-    }
-  });
-  H.colslot.assign((size_t)cols + 1, 0);
-  for (uint32_t c = 0; c < cols; ++c) H.colslot[c + 1] = H.colslot[c] + nchunk_pc[c];
-  std::vector<uint64_t> gbase((size_t)P + 1, 0);
-  for (uint32_t p = 0; p < P; ++p) gbase[p + 1] = gbase[p] + ngroups_p[p];
-  const uint64_t ngroups = gbase[P];
-  if (ngroups > 0xFFFFFFF0ull) throw std::invalid_argument("too many groups");
-  H.bseg.assign(ngroups + 1, 0);
-  H.bslot.assign(ngroups, 0);
-  H.brow.assign(nnz, 0);
+  H.chunk_piece.assign((size_t)nchunks + 1, 0);
+  H.chunk_elem.assign((size_t)nchunks + 1, 0);
+  H.boff.assign((size_t)nchunks * (kBoxVox + 1), 0);
+  H.bpc.assign(nnz, 0);
   H.bval.assign(nnz, 0.f);
-  H.cgroup.assign((size_t)nchunks + 1, 0);
-  // phase B (threads over panels): group offsets and slots in (chunk, voxel)
-  // order, then the entries of every segment in row order
   parallel_for(TP, [&](int t) {
-    std::vector<uint64_t> gpos;
+    std::vector<uint32_t> cur(kBoxVox + 1);
     for (uint32_t p = (uint32_t)t; p < P; p += (uint32_t)TP) {
-      const uint32_t pv = pstart[p + 1] - pstart[p];
-      const uint32_t nk = H.pchunk[p + 1] - H.pchunk[p];
-      const std::vector<uint32_t>& g = gcount[p];
-      gpos.assign((size_t)nk * pv, 0);
-      std::vector<uint32_t> rank(pv, 0);
-      uint64_t gi = gbase[p], e = pel[p];
-      for (uint32_t k = 0; k < nk; ++k) {
-        H.cgroup[H.pchunk[p] + k] = (uint32_t)gi;
-        for (uint32_t v = 0; v < pv; ++v) {
-          const uint32_t n = g[(size_t)k * pv + v];
-          if (!n) continue;
-          gpos[(size_t)k * pv + v] = e;
-          for (uint32_t done = 0; done < n; done += SP) {
-            H.bseg[gi] = (uint32_t)(e + done);
-            H.bslot[gi] = H.colslot[pstart[p] + v] + rank[v]++;
-            ++gi;
+      uint32_t k = H.pchunk[p];
+      for (uint32_t s = H.pseg[p]; s < H.pseg[p + 1]; ++k) {
+        const uint32_t s1 = chunk_end(s, H.pseg[p + 1]);
+        const uint32_t e0 = rseg[s], e1 = rseg[s1];
+        H.chunk_piece[k] = s;
+        H.chunk_elem[k] = e0;
+        // elements per voxel, then their offsets; elements placed voxel by
+        // voxel, pieces (rows) ascending inside a voxel
+        std::fill(cur.begin(), cur.end(), 0u);
+        for (uint32_t e = e0; e < e1; ++e) cur[rcol[e] + 1]++;
+        for (uint32_t v = 0; v < kBoxVox; ++v) cur[v + 1] += cur[v];
+        uint16_t* off = &H.boff[(size_t)k * (kBoxVox + 1)];
+        for (uint32_t v = 0; v <= kBoxVox; ++v) off[v] = (uint16_t)cur[v];
+        for (uint32_t q = s; q < s1; ++q)
+          for (uint32_t e = rseg[q]; e < rseg[q + 1]; ++e) {
+            const uint32_t at = e0 + cur[rcol[e]]++;
+            H.bpc[at] = (uint16_t)(q - s);
+            H.bval[at] = rval[e];
           }
-          e += n;
-        }
-      }
-      for (uint32_t s = H.pseg[p]; s < H.pseg[p + 1]; ++s) {
-        const uint32_t k = (s - H.pseg[p]) / C;
-        const uint32_t r = segrow[s];
-        for (uint32_t q = H.fseg[s]; q < H.fseg[s + 1]; ++q) {
-          const uint64_t at = gpos[(size_t)k * pv + H.fcol[q]]++;
-          H.brow[at] = r;
-          H.bval[at] = H.fval[q];
-        }
+        s = s1;
       }
     }
   });
-  H.cgroup[nchunks] = (uint32_t)ngroups;
-  H.bseg[ngroups] = (uint32_t)nnz;
+  H.chunk_piece[nchunks] = (uint32_t)nseg;
+  H.chunk_elem[nchunks] = (uint32_t)nnz;
+
+  // launch order: boxes by decreasing element count (largest first)
+  H.border.resize(P);
+  std::iota(H.border.begin(), H.border.end(), 0u);
+  std::stable_sort(H.border.begin(), H.border.end(), [&](uint32_t a, uint32_t b) {
+    return pel[a + 1] - pel[a] > pel[b + 1] - pel[b];
+  });
 }
 
 }  // namespace mlem

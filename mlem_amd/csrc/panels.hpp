@@ -1,27 +1,31 @@
 // panels.hpp — the system matrix as a two-dimensional tiling for products
 // with many right-hand sides (host-side construction, no GPU needed).
 //
-// A product with W interleaved right-hand sides gathers one W*4-byte run per
-// matrix element (128 bytes at W = 32). Over the whole matrix those runs are
-// tens of GB, far more than the element stream, and they only stay cheap if
-// the vector runs that the workgroups of one XCD gather at a time fit in its
-// L2. Two tilings keep them there:
+// A product with W interleaved right-hand sides needs one W*4-byte run of
+// the dense operand per matrix element (128 bytes at W = 32). Gathered from
+// the caches element by element, those runs are tens of GB per product. The
+// tiling cuts the voxel grid into boxes small enough that one box's slice of
+// a W-wide block fits in a workgroup's LDS (16x16x4 = 1024 voxels, 128 KB at
+// W = 32), so that the runs are read from LDS and reused by every row that
+// crosses the box:
 //
-//   forward (f = A X): the voxels are grouped into panels, boxes of the grid
-//   (32x32x16 voxels: 2 MB of X at W = 32). Every row is cut into segments,
-//   one per panel it crosses (about 4.4 per row on the 128x128x48 grid); the
-//   segments of one panel are stored together, rows ascending, with their
-//   columns as 16-bit offsets inside the panel. A panel's segments gather only
-//   that panel's slice of X. Each segment's dot product is written to its own
-//   slot; the slots of one row are contiguous (row-major), so a second pass
-//   adds them in panel order.
+//   pieces: every row is cut into pieces, one per box it crosses (about 10
+//   per row on the 128x128x48 grid, 12 elements each); a piece's elements
+//   keep their file order and carry their column as a 16-bit offset inside
+//   the box.
 //
-//   backward (u = A^T R): inside each panel the segments (rows ascending) are
-//   split into chunks of `chunk_rows` segments; the chunk's rows' runs of R
-//   are the gathered working set (2 MB at W = 32, 16384 rows). For every
-//   (panel, chunk, voxel) group the elements are stored as {row, value},
-//   rows ascending; each group's dot product goes to a slot, the slots of one
-//   voxel contiguous, added in chunk order by a second pass.
+//   forward (f = A X): a workgroup loads its box's slice of X into LDS and
+//   writes one W-wide dot product per piece into the piece's slot. The slots
+//   of one row are contiguous, in box order, so a second pass adds them in a
+//   fixed order. Inside a box the pieces are stored longest first, so the
+//   lanes of a wave get pieces of about the same length, and staged in LDS
+//   in chunks (at most 256 pieces and 4096 elements).
+//
+//   backward (u = A^T R): a workgroup holds its box's voxel sums in
+//   registers and walks the box's pieces (rows ascending) in chunks: the
+//   chunk's runs of R are staged in LDS, then every voxel adds its elements
+//   of the chunk (chunk order, then rows ascending). Every voxel's sum is
+//   complete at the end, one slot per voxel.
 //
 // Sums are deterministic (fixed order), not the sequential order of the
 // reference driver (mlem_seq/seqcsr4mlem.cpp:69-120).
@@ -32,49 +36,57 @@
 
 namespace mlem {
 
-// Longest segment (forward) or group (backward): longer pieces are split into
-// several, each with its own slot, so that every segment fits a staged tile.
-constexpr uint32_t kMaxSegment = 512;
+constexpr uint32_t kBoxVox = 1024;      // voxels per box (at most)
+constexpr uint32_t kMaxSegment = 512;   // longer pieces are split, each with its own slot
+constexpr uint32_t kChunkPieces = 512;   // backward: pieces staged per chunk (at most)
+constexpr uint32_t kChunkElems = 8192;   // backward: elements staged per chunk (at most)
+constexpr uint32_t kFChunkPieces = 256;  // forward: pieces staged per chunk (at most)
+constexpr uint32_t kFChunkElems = 4096;  // forward: elements staged per chunk (at most)
 
 struct HostPanels {
   uint32_t rows = 0, cols = 0;
   uint64_t nnz = 0;
-  uint32_t npanels = 0;
-  uint32_t chunk_rows = 0;
-  // column order of the tiling ("panel order"): panels one after the other,
-  // the voxels of a panel in 4x4x2 bricks
+  uint32_t npanels = 0;  // boxes
+  // column order of the tiling ("panel order"): boxes one after the other
   std::vector<uint32_t> pc_of_dev;    // [cols] panel-order column of each device column
-  std::vector<uint32_t> pstart;       // [npanels+1] first panel-order column of each panel
-  // forward: segments (row, panel), panel-major, rows ascending
-  std::vector<uint32_t> fseg;         // [nseg+1] first element of each segment
-  std::vector<uint32_t> fslot;        // [nseg] row-major slot of each segment
-  std::vector<float> fval;            // [nnz] values, segment-major
-  std::vector<uint16_t> fcol;         // [nnz] column inside the segment's panel
+  std::vector<uint32_t> pstart;       // [npanels+1] first panel-order column of each box
+  // forward: pieces box-major, longest first inside a box
+  std::vector<uint32_t> pseg;         // [npanels+1] first piece of each box
+  std::vector<uint32_t> fseg;         // [nseg+1] first element of each piece
+  std::vector<uint32_t> fslot;        // [nseg] row-major slot of each piece
+  std::vector<float> fval;            // [nnz] values, piece-major
+  std::vector<uint16_t> fcol;         // [nnz] column inside the piece's box
   std::vector<uint32_t> rowslot;      // [rows+1] first slot of each device row
-  std::vector<uint32_t> pseg;         // [npanels+1] first segment of each panel
-  // backward: groups (panel, chunk, voxel), rows ascending inside a group
-  std::vector<uint32_t> bseg;         // [ngroups+1] first entry of each group
-  std::vector<uint32_t> bslot;        // [ngroups] voxel-major slot of each group
-  std::vector<uint32_t> brow;         // [nnz] device row of each entry
-  std::vector<float> bval;            // [nnz] value of each entry
-  std::vector<uint32_t> colslot;      // [cols+1] first slot of each panel-order column
-  std::vector<uint32_t> pchunk;       // [npanels+1] first chunk (global numbering) of each panel
-  std::vector<uint32_t> cgroup;       // [nchunks+1] first group of each chunk
+  std::vector<uint32_t> pfchunk;      // [npanels+1] first forward chunk of each box
+  std::vector<uint32_t> fchunk_piece; // [nfchunks+1] first piece of each forward chunk
+  // backward: pieces box-major, rows ascending inside a box, in chunks
+  std::vector<uint32_t> brow;         // [nseg] device row of each piece
+  std::vector<uint32_t> pchunk;       // [npanels+1] first chunk of each box
+  std::vector<uint32_t> chunk_piece;  // [nchunks+1] first piece of each chunk
+  std::vector<uint32_t> chunk_elem;   // [nchunks+1] first element of each chunk
+  std::vector<uint16_t> boff;         // [nchunks*(kBoxVox+1)] element offset of each
+                                      //  in-box voxel, relative to the chunk's first
+  std::vector<uint16_t> bpc;          // [nnz] piece of each element, inside its chunk
+  std::vector<float> bval;            // [nnz] value of each element
+  std::vector<uint32_t> border;       // [npanels] launch order: decreasing element count
 };
 
 // Panel-order column of every file column (voxel x + nx*y + nx*ny*z,
 // helper_files_common/scannerconfig.cpp:30-34) for a grid of nx*ny*nz ==
-// cols voxels, with the first column of every panel; without a grid (any
-// zero) columns keep their order and panels are runs of 16384 columns.
+// cols voxels, with the first column of every box; without a grid (any
+// zero) columns keep their order and boxes are runs of kBoxVox columns.
+// boxgrid receives the number of boxes along x, y, z (box p at x = p % nbx,
+// y = (p / nbx) % nby, z = p / (nbx * nby)), or {P, 1, 1} without a grid.
 void panel_order(uint32_t cols, const uint32_t grid[3], std::vector<uint32_t>* pc_of_file,
-                 std::vector<uint32_t>* pstart);
+                 std::vector<uint32_t>* pstart, uint32_t boxgrid[3]);
 
 // Build the tiling from the device-order CSR (row pointers, {column, value
 // bits} records) and the panel-order column of every device column.
-// threads = 0: hardware concurrency (at most 16).
+// boxgrid: as returned by panel_order. threads = 0: hardware concurrency
+// (at most 16).
 void build_host_panels(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t* rowptr,
                        const uint32_t* csr2 /* [2*nnz] */, const std::vector<uint32_t>& pc_of_dev,
-                       const std::vector<uint32_t>& pstart, uint32_t chunk_rows, int threads,
-                       HostPanels* out);
+                       const std::vector<uint32_t>& pstart, const uint32_t boxgrid[3],
+                       int threads, HostPanels* out);
 
 }  // namespace mlem

@@ -19,11 +19,11 @@
 // every product is the RHS-block SpMM of spmm.hpp, each frame's sums
 // sequential fp32 (deterministic; the same for any batch composition; for
 // integer counts the forward pass is bit-identical to mlem()).
-// MLEM_MODE_FAST: every product runs through the matrix's panel tiling
-// (panel.hip): partial sums per (row, panel) segment or (panel, row chunk,
-// voxel) group, added in a fixed order by k_slot_sums before the epilogue;
-// column vectors are then held in panel order. Deterministic for a given
-// block width.
+// MLEM_MODE_FAST: every product runs through the matrix's box tiling
+// (panel.hip): the forward's sums per (row, box) piece are added in a fixed
+// order by k_slot_sums before the epilogue, the backward's sums are complete
+// per voxel (one slot each); column vectors are then held in panel order.
+// Deterministic for a given block width.
 //
 // Memory per block of W frames: ceil(K/C) checkpoints of cols*W floats plus
 // C iterations of (2 rows + 2 cols)*W floats; C = round(sqrt(K)) by default.
@@ -125,6 +125,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm(const uint32_t* __restrict__ pt
 // sum to the epilogue.
 template <int W, class Epi>
 __global__ __launch_bounds__(kBlock) void k_slot_sums(const uint32_t* __restrict__ first,
+                                                      const uint32_t* __restrict__ index,
                                                       uint32_t n, const float* __restrict__ part,
                                                       Epi epi) {
   const uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
@@ -132,7 +133,7 @@ __global__ __launch_bounds__(kBlock) void k_slot_sums(const uint32_t* __restrict
   const uint32_t r = (uint32_t)(i / W), l = (uint32_t)(i % W);
   const uint32_t s0 = first[r], s1 = first[r + 1];
   float acc = 0.f;
-  for (uint32_t k = s0; k < s1; ++k) acc = acc + part[(size_t)k * W + l];
+  for (uint32_t k = s0; k < s1; ++k) acc = acc + part[(size_t)(index ? index[k] : k) * W + l];
   epi(r, l, acc);
 }
 
@@ -268,7 +269,8 @@ void spmm_launch(mlem_tape* T, bool csc, const float* v, const Epi& epi) {
     else
       launch_panel_fwd<W>(P, v, part, T->grid, m.stream);
     hipLaunchKernelGGL((k_slot_sums<W, Epi>), dim3(blocks_for((uint64_t)n * W)), dim3(kBlock), 0,
-                       m.stream, csc ? P.colslot : P.rowslot, n, part, epi);
+                       m.stream, csc ? P.colslot : P.rowslot,
+                       (!csc && panel_slots_by_box(W)) ? P.rowpiece : nullptr, n, part, epi);
   } else if (csc) {
     hipLaunchKernelGGL((k_spmm<W, Epi>), dim3(T->grid), dim3(kBlock), 0, m.stream, m.colptr, m.csc,
                        T->tiles_b, T->nt_b, v, epi);
