@@ -83,6 +83,8 @@ __device__ __forceinline__ float4 across_e(float4 v) {
 }
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // 16-byte store that does not allocate in the caches (partial sums are read
 // once, by the next kernel).
@@ -122,10 +124,10 @@ __device__ __forceinline__ uint32_t bcast(uint32_t x) {
 }
 
 // Adds a batch of 8 staged elements [i, i + 8) of a list ending at e to the
-// lane's sums: lane g8 = ei*L + q of its run of 8 reads element i + g8 (past
-// e: weight 0) from LDS, and share ei takes elements i + ei*L + m, m < L,
-// from its run of L lanes; per element the lane reads its 16-byte quad of
-// the operand row idx from LDS. The reads go out in groups of up to four
+// lane's sums: lane g8 = ei*L + q of its run of 8 reads element i + g8 from
+// LDS, and share ei takes elements i + ei*L + m, m < L, from its run of L
+// lanes; per element the lane reads its 16-byte quad of the operand row idx
+// from LDS (past e: weight 0). The reads go out in groups of up to four
 // before their multiply-adds.
 template <uint32_t L, uint32_t m, uint32_t n, uint32_t N>
 __device__ __forceinline__ void gather_rec(uint32_t idx, uint32_t vbits,
@@ -169,30 +171,30 @@ __global__ __launch_bounds__(kBoxThreads) void k_box_fwd(
     const float* __restrict__ fval, const uint16_t* __restrict__ fcol,
     const float* __restrict__ X, float* __restrict__ part) {
   constexpr uint32_t L = Quad<W>::L, E = Quad<W>::E, G = Quad<W>::G;
-  constexpr uint32_t kPer = kFChunkElems / kBoxThreads;
   constexpr uint32_t XJ = kBoxVox * L / kBoxThreads;
-  static_assert(kPer * kBoxThreads == kFChunkElems && kFChunkPieces < kBoxThreads, "staging");
+  static_assert(kFChunkElems == 4 * kBoxThreads && kFChunkPieces < kBoxThreads &&
+                kFAlign % 4 == 0, "staging: four elements per thread, 16-byte aligned chunks");
   __shared__ float4 xs[kBoxVox * L];
-  __shared__ float sval[kFChunkElems];
-  __shared__ uint16_t scol[kFChunkElems];
+  __shared__ float4 sval4[kFChunkElems / 4];
+  __shared__ uint2 scol4[kFChunkElems / 4];
+  const float* __restrict__ sval = reinterpret_cast<const float*>(sval4);
+  const uint16_t* __restrict__ scol = reinterpret_cast<const uint16_t*>(scol4);
   __shared__ uint16_t soff[kBoxThreads + 1];  // soff[s + 1]: end of piece s in the chunk
   __shared__ uint32_t sslot[kBoxThreads];
   const uint32_t box = border[blockIdx.x];
   const uint32_t c0 = pstart[box], nv = pstart[box + 1] - c0;
   const uint32_t t = threadIdx.x, q = t % L, ei = (t / L) % E, grp = t / 8, g8 = t % 8;
   const uint32_t k0 = pfchunk[box], k1 = pfchunk[box + 1];
-  float rv[kPer];
-  uint32_t rc[kPer];
+  f32x4 rv;
+  u32x2 rc;
   uint32_t rend = 0, rslot = 0;
   auto fetch = [&](uint32_t k) __attribute__((always_inline)) {  // chunk k into registers
     const uint2 a = fchunk[k], b = fchunk[k + 1];
-    const uint32_t ne = b.y - a.y, np = b.x - a.x;
-#pragma unroll
-    for (uint32_t j = 0; j < kPer; ++j) {  // streamed once: non-temporal
-      const uint32_t i = a.y + min(t + j * kBoxThreads, ne - 1);
-      rv[j] = __builtin_nontemporal_load(fval + i);
-      rc[j] = __builtin_nontemporal_load(fcol + i);
-    }
+    const uint32_t np = b.x - a.x;
+    // elements 4t .. 4t+3 of the chunk (the arrays end with a chunk of zeros);
+    // streamed once: non-temporal
+    rv = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(fval + a.y) + t);
+    rc = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(fcol + a.y) + t);
     const uint32_t sp = a.x + min(t, np - 1);
     rend = fseg[sp + 1];
     if constexpr (!kByBox) rslot = fslot[sp];
@@ -213,16 +215,17 @@ __global__ __launch_bounds__(kBoxThreads) void k_box_fwd(
     const uint2 a = fchunk[k], b = fchunk[k + 1];
     const uint32_t np = b.x - a.x;
     __syncthreads();  // the previous chunk's readers are done with the LDS arrays
-#pragma unroll
-    for (uint32_t j = 0; j < kPer; ++j) {
-      sval[t + j * kBoxThreads] = rv[j];
-      scol[t + j * kBoxThreads] = (uint16_t)rc[j];
-    }
+    sval4[t] = make_float4(rv.x, rv.y, rv.z, rv.w);
+    scol4[t] = make_uint2(rc.x, rc.y);
     soff[t + 1] = (uint16_t)(rend - a.y);
     sslot[t] = kByBox ? a.x + t : rslot;
     __syncthreads();
     if (k + 1 < k1) fetch(k + 1);
-    for (uint32_t sg = grp; sg < np; sg += G) {
+    // pieces are longest first: rounds of G pieces go to the groups in snake
+    // order, so that the waves get about the same work
+    for (uint32_t r0 = 0; r0 < np; r0 += G) {
+      const uint32_t sg = r0 + (((r0 / G) & 1u) ? G - 1 - grp : grp);
+      if (sg >= np) continue;
       const uint32_t e = soff[sg + 1];
       float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
       for (uint32_t i = soff[sg]; i < e; i += 8) {
@@ -251,12 +254,14 @@ __global__ __launch_bounds__(kBoxThreads) __attribute__((amdgpu_waves_per_eu(4, 
     const uint16_t* __restrict__ bpc, const float* __restrict__ bval,
     const float* __restrict__ R, float* __restrict__ U) {
   constexpr uint32_t L = Quad<W>::L, E = Quad<W>::E, V = kGroupVox;
-  constexpr uint32_t kPer = kChunkElems / kBoxThreads;
   constexpr uint32_t kRq = (kChunkPieces * L + kBoxThreads - 1) / kBoxThreads;  // R quads
-  static_assert(kPer * kBoxThreads == kChunkElems && kBoxVox == kBoxThreads, "staging");
+  static_assert(kChunkElems == 8 * kBoxThreads && kBoxVox == kBoxThreads && kBAlign % 8 == 0,
+                "staging: eight elements per thread, 16-byte aligned chunks");
   __shared__ float4 rs[kRq * kBoxThreads];
-  __shared__ float sval[kChunkElems];
-  __shared__ uint16_t spc[kChunkElems];
+  __shared__ float4 sval4[kChunkElems / 4];
+  __shared__ uint4 spc4[kChunkElems / 8];
+  const float* __restrict__ sval = reinterpret_cast<const float*>(sval4);
+  const uint16_t* __restrict__ spc = reinterpret_cast<const uint16_t*>(spc4);
   __shared__ uint16_t soff[kBoxVox + 1];  // soff[v + 1]: end of voxel v's elements
   const uint32_t box = border[blockIdx.x];
   const uint32_t c0 = pstart[box], nv = pstart[box + 1] - c0;
@@ -268,8 +273,8 @@ __global__ __launch_bounds__(kBoxThreads) __attribute__((amdgpu_waves_per_eu(4, 
   typedef uint32_t rr_t __attribute__((ext_vector_type(kRq)));
   rq_t rR;
   rr_t rrow;
-  float rv[kPer];
-  uint32_t rp[kPer];
+  f32x4 rva, rvb;
+  u32x4 rp;
   uint32_t rend = 0;
   auto rows_of = [&](uint32_t k) __attribute__((always_inline)) {  // rows of chunk k's R quads
     const uint2 a = chunk[k], b = chunk[k + 1];
@@ -279,8 +284,7 @@ __global__ __launch_bounds__(kBoxThreads) __attribute__((amdgpu_waves_per_eu(4, 
 
   };
   auto fetch = [&](uint32_t k) __attribute__((always_inline)) {  // chunk k (its rows in rrow)
-    const uint2 a = chunk[k], b = chunk[k + 1];
-    const uint32_t ne = b.y - a.y;
+    const uint2 a = chunk[k];
 #pragma unroll
     for (uint32_t j = 0; j < kRq; ++j) {
       const float4 g = R4[(size_t)rrow[j] * L + (t + j * kBoxThreads) % L];
@@ -289,12 +293,11 @@ __global__ __launch_bounds__(kBoxThreads) __attribute__((amdgpu_waves_per_eu(4, 
       rR[4 * j + 2] = g.z;
       rR[4 * j + 3] = g.w;
     }
-#pragma unroll
-    for (uint32_t j = 0; j < kPer; ++j) {  // streamed once: non-temporal
-      const uint32_t i = a.y + min(t + j * kBoxThreads, ne - 1);
-      rv[j] = __builtin_nontemporal_load(bval + i);
-      rp[j] = __builtin_nontemporal_load(bpc + i);
-    }
+    // elements 8t .. 8t+7 of the chunk (the arrays end with a chunk of
+    // zeros); streamed once: non-temporal
+    rva = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(bval + a.y) + 2 * t);
+    rvb = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(bval + a.y) + 2 * t + 1);
+    rp = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(bpc + a.y) + t);
     rend = boff[(size_t)k * (kBoxVox + 1) + t + 1];
   };
   if (k0 < k1) {
@@ -311,11 +314,9 @@ __global__ __launch_bounds__(kBoxThreads) __attribute__((amdgpu_waves_per_eu(4, 
 #pragma unroll
     for (uint32_t j = 0; j < kRq; ++j)
       rs[t + j * kBoxThreads] = make_float4(rR[4 * j], rR[4 * j + 1], rR[4 * j + 2], rR[4 * j + 3]);
-#pragma unroll
-    for (uint32_t j = 0; j < kPer; ++j) {
-      sval[t + j * kBoxThreads] = rv[j];
-      spc[t + j * kBoxThreads] = (uint16_t)rp[j];
-    }
+    sval4[2 * t] = make_float4(rva.x, rva.y, rva.z, rva.w);
+    sval4[2 * t + 1] = make_float4(rvb.x, rvb.y, rvb.z, rvb.w);
+    spc4[t] = make_uint4(rp.x, rp.y, rp.z, rp.w);
     soff[t + 1] = (uint16_t)rend;
     __syncthreads();
     if (k + 1 < k1) {
@@ -390,7 +391,11 @@ const PanelFormat& ensure_panels(DevMatrix& m) {
   std::vector<uint32_t> pc_of_file, pstart;
   uint32_t boxgrid[3] = {0, 0, 0};
   const bool has_grid = (uint64_t)m.grid[0] * m.grid[1] * m.grid[2] == m.cols && m.cols;
-  panel_order(m.cols, has_grid ? m.grid : nullptr, &pc_of_file, &pstart, boxgrid);
+  std::vector<uint32_t> count_dev(m.cols, 0), count_file(m.cols);
+  for (uint64_t k = 0; k < m.nnz; ++k) count_dev[csr2[2 * k]]++;
+  for (uint32_t c = 0; c < m.cols; ++c) count_file[c] = count_dev[colmap[c]];
+  panel_order(m.cols, has_grid ? m.grid : nullptr, count_file.data(), &pc_of_file, &pstart,
+              boxgrid);
   std::vector<uint32_t> pc_of_dev(m.cols), dev_of_pc(m.cols);
   for (uint32_t c = 0; c < m.cols; ++c) pc_of_dev[colmap[c]] = pc_of_file[c];
   for (uint32_t d = 0; d < m.cols; ++d) dev_of_pc[pc_of_dev[d]] = d;

@@ -36,44 +36,57 @@ void parallel_for(int nthreads, F&& f) {
 
 }  // namespace
 
-void panel_order(uint32_t cols, const uint32_t grid[3], std::vector<uint32_t>* pc_of_file,
-                 std::vector<uint32_t>* pstart, uint32_t boxgrid[3]) {
+void panel_order(uint32_t cols, const uint32_t grid[3], const uint32_t* count,
+                 std::vector<uint32_t>* pc_of_file, std::vector<uint32_t>* pstart,
+                 uint32_t boxgrid[3]) {
   pc_of_file->assign(cols, 0);
   pstart->clear();
   const uint64_t nx = grid ? grid[0] : 0, ny = grid ? grid[1] : 0, nz = grid ? grid[2] : 0;
+  // box of every file column, boxes numbered x fastest; file columns of every box
+  std::vector<uint32_t> box_of(cols);
+  uint32_t nboxes = 0;
   if (!nx || !ny || !nz || nx * ny * nz != cols) {
-    for (uint32_t c = 0; c < cols; ++c) (*pc_of_file)[c] = c;
-    for (uint64_t p = 0; p < cols; p += kBoxVox) pstart->push_back((uint32_t)p);
-    pstart->push_back(cols);
-    boxgrid[0] = (uint32_t)pstart->size() - 1;
+    for (uint32_t c = 0; c < cols; ++c) box_of[c] = c / kBoxVox;
+    nboxes = (uint32_t)((cols + kBoxVox - 1) / kBoxVox);
+    boxgrid[0] = nboxes;
     boxgrid[1] = boxgrid[2] = 1;
-    return;
-  }
-  const uint64_t nbx = (nx + kBoxX - 1) / kBoxX, nby = (ny + kBoxY - 1) / kBoxY;
-  boxgrid[0] = (uint32_t)nbx;
-  boxgrid[1] = (uint32_t)nby;
-  boxgrid[2] = (uint32_t)((nz + kBoxZ - 1) / kBoxZ);
-  // sort key: box (x fastest), then the voxel inside the box (x fastest);
-  // ranks of the keys are the panel-order columns
-  std::vector<uint64_t> key(cols);
-  for (uint64_t c = 0; c < cols; ++c) {
-    const uint64_t x = c % nx, y = (c / nx) % ny, z = c / (nx * ny);
-    const uint64_t p = x / kBoxX + nbx * (y / kBoxY + nby * (z / kBoxZ));
-    const uint64_t in = x % kBoxX + kBoxX * (y % kBoxY + kBoxY * (z % kBoxZ));
-    key[c] = (p << 32) | in;
-  }
-  std::vector<uint32_t> order(cols);
-  std::iota(order.begin(), order.end(), 0u);
-  std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
-  uint64_t prev = ~0ull;
-  for (uint32_t r = 0; r < cols; ++r) {
-    const uint32_t c = order[r];
-    (*pc_of_file)[c] = r;
-    const uint64_t p = key[c] >> 32;
-    if (p != prev) {
-      pstart->push_back(r);
-      prev = p;
+  } else {
+    const uint64_t nbx = (nx + kBoxX - 1) / kBoxX, nby = (ny + kBoxY - 1) / kBoxY;
+    const uint64_t nbz = (nz + kBoxZ - 1) / kBoxZ;
+    for (uint64_t c = 0; c < cols; ++c) {
+      const uint64_t x = c % nx, y = (c / nx) % ny, z = c / (nx * ny);
+      box_of[c] = (uint32_t)(x / kBoxX + nbx * (y / kBoxY + nby * (z / kBoxZ)));
     }
+    nboxes = (uint32_t)(nbx * nby * nbz);
+    boxgrid[0] = (uint32_t)nbx;
+    boxgrid[1] = (uint32_t)nby;
+    boxgrid[2] = (uint32_t)nbz;
+  }
+  std::vector<uint32_t> first(nboxes + 1, 0), members(cols);
+  for (uint32_t c = 0; c < cols; ++c) first[box_of[c] + 1]++;
+  for (uint32_t p = 0; p < nboxes; ++p) first[p + 1] += first[p];
+  {
+    std::vector<uint32_t> cur(first.begin(), first.end() - 1);
+    for (uint32_t c = 0; c < cols; ++c) members[cur[box_of[c]]++] = c;  // file order in a box
+  }
+  constexpr uint32_t kGroup = 8, kGroups = kBoxVox / kGroup;
+  std::vector<uint32_t> pos;
+  for (uint32_t p = 0; p < nboxes; ++p) {
+    pstart->push_back(first[p]);
+    const uint32_t nv = first[p + 1] - first[p];
+    uint32_t* m = &members[first[p]];
+    if (count)  // most elements first (ties: file order)
+      std::stable_sort(m, m + nv, [&](uint32_t a, uint32_t b) { return count[a] > count[b]; });
+    // positions in deal order: round j gives every group g its position 8g + j,
+    // groups in snake order (ascending in even rounds, descending in odd ones)
+    const uint32_t groups = std::min(kGroups, (nv + kGroup - 1) / kGroup);
+    pos.clear();
+    for (uint32_t j = 0; j < kGroup; ++j)
+      for (uint32_t i = 0; i < groups; ++i) {
+        const uint32_t g = (j % 2 == 0) ? i : groups - 1 - i;
+        if (g * kGroup + j < nv) pos.push_back(g * kGroup + j);
+      }
+    for (uint32_t r = 0; r < nv; ++r) (*pc_of_file)[m[r]] = first[p] + pos[r];
   }
   pstart->push_back(cols);
 }
@@ -225,21 +238,63 @@ void build_host_panels(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_
 
   const int TP = std::max(1, std::min<int>(T, (int)std::max<uint32_t>(P, 1)));
   // ---- forward: inside every box the pieces longest first (stable)
-  H.fseg.assign(nseg + 1, 0);
-  H.fslot.assign(nseg, 0);
-  H.fval.assign(nnz, 0.f);
-  H.fcol.assign(nnz, 0);
+  // Chunks of both directions start at multiples of kFAlign / kBAlign
+  // elements, so that their elements load in 16-byte pieces; the gaps hold
+  // zero values (column / piece 0), and the arrays end with a chunk's worth
+  // of zeros so that whole-chunk loads stay inside them.
+  auto align_up = [](uint64_t x, uint64_t a) { return (x + a - 1) / a * a; };
+  // ---- forward: inside every box the pieces longest first (stable), cut
+  // greedily into chunks of at most kFChunkPieces pieces / kFChunkElems
+  // elements
+  std::vector<std::vector<uint32_t>> forder(P), fcut(P);
+  std::vector<uint64_t> fsize(P, 0);
   parallel_for(TP, [&](int t) {
-    std::vector<uint32_t> idx;
     for (uint32_t p = (uint32_t)t; p < P; p += (uint32_t)TP) {
       const uint32_t s0 = H.pseg[p], s1 = H.pseg[p + 1];
+      std::vector<uint32_t>& idx = forder[p];
       idx.resize(s1 - s0);
       std::iota(idx.begin(), idx.end(), s0);
       std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
         return rseg[a + 1] - rseg[a] > rseg[b + 1] - rseg[b];
       });
-      uint64_t e = pel[p];
-      for (uint32_t i = 0; i < s1 - s0; ++i) {
+      uint64_t size = 0;
+      for (uint32_t i = 0; i < s1 - s0;) {
+        fcut[p].push_back(i);
+        uint32_t n = 0, j = i;
+        while (j < s1 - s0 && j - i < kFChunkPieces &&
+               n + (rseg[idx[j] + 1] - rseg[idx[j]]) <= kFChunkElems)
+          n += rseg[idx[j] + 1] - rseg[idx[j]], ++j;
+        size += align_up(n, kFAlign);
+        i = j;
+      }
+      fsize[p] = size;
+    }
+  });
+  std::vector<uint64_t> fbase((size_t)P + 1, 0);
+  H.pfchunk.assign((size_t)P + 1, 0);
+  for (uint32_t p = 0; p < P; ++p) {
+    fbase[p + 1] = fbase[p] + fsize[p];
+    H.pfchunk[p + 1] = H.pfchunk[p] + (uint32_t)fcut[p].size();
+  }
+  const uint64_t nfel = fbase[P];
+  if (nfel + kFChunkElems > 0xFFFFFFFFull) throw std::invalid_argument("too many elements");
+  H.fseg.assign(nseg + 1, 0);
+  H.fslot.assign(nseg, 0);
+  H.fval.assign(nfel + kFChunkElems, 0.f);
+  H.fcol.assign(nfel + kFChunkElems, 0);
+  H.fchunk_piece.assign((size_t)H.pfchunk[P] + 1, 0);
+  parallel_for(TP, [&](int t) {
+    for (uint32_t p = (uint32_t)t; p < P; p += (uint32_t)TP) {
+      const uint32_t s0 = H.pseg[p], np = H.pseg[p + 1] - s0;
+      const std::vector<uint32_t>& idx = forder[p];
+      uint64_t e = fbase[p];
+      uint32_t k = H.pfchunk[p];
+      for (uint32_t i = 0, c = 0; i < np; ++i) {
+        if (c < fcut[p].size() && fcut[p][c] == i) {  // a chunk starts: align
+          e = fbase[p] + align_up(e - fbase[p], kFAlign);
+          H.fchunk_piece[k++] = s0 + i;
+          ++c;
+        }
         const uint32_t s = idx[i], n = rseg[s + 1] - rseg[s];
         H.fseg[s0 + i] = (uint32_t)e;
         H.fslot[s0 + i] = rslot[s];
@@ -249,29 +304,14 @@ void build_host_panels(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_
       }
     }
   });
-  H.fseg[nseg] = (uint32_t)nnz;
-  // forward chunks: greedy over the length-sorted pieces of every box
-  auto fchunk_end = [&](uint32_t s, uint32_t s1) {
-    uint32_t e = s + 1;
-    while (e < s1 && e - s < kFChunkPieces && H.fseg[e + 1] - H.fseg[s] <= kFChunkElems) ++e;
-    return e;
-  };
-  H.pfchunk.assign((size_t)P + 1, 0);
-  for (uint32_t p = 0; p < P; ++p) {
-    uint32_t n = 0;
-    for (uint32_t s = H.pseg[p]; s < H.pseg[p + 1]; s = fchunk_end(s, H.pseg[p + 1])) ++n;
-    H.pfchunk[p + 1] = H.pfchunk[p] + n;
-  }
-  H.fchunk_piece.assign((size_t)H.pfchunk[P] + 1, 0);
-  for (uint32_t p = 0; p < P; ++p) {
-    uint32_t k = H.pfchunk[p];
-    for (uint32_t s = H.pseg[p]; s < H.pseg[p + 1]; s = fchunk_end(s, H.pseg[p + 1]))
-      H.fchunk_piece[k++] = s;
-  }
+  // a piece ends where the next one starts (the last of a chunk includes the
+  // zero gap before the next chunk)
+  H.fseg[nseg] = (uint32_t)nfel;
   H.fchunk_piece[H.pfchunk[P]] = (uint32_t)nseg;
 
   // ---- backward: chunks of the row-ordered pieces of every box
   std::vector<uint32_t> nck(P, 0);
+  std::vector<uint64_t> bsize(P, 0);
   auto chunk_end = [&](uint32_t s, uint32_t s1) {  // greedy: pieces and elements bounded
     uint32_t e = s + 1;
     while (e < s1 && e - s < kChunkPieces && rseg[e + 1] - rseg[s] <= kChunkElems) ++e;
@@ -279,25 +319,36 @@ void build_host_panels(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_
   };
   parallel_for(TP, [&](int t) {
     for (uint32_t p = (uint32_t)t; p < P; p += (uint32_t)TP)
-      for (uint32_t s = H.pseg[p]; s < H.pseg[p + 1]; s = chunk_end(s, H.pseg[p + 1])) nck[p]++;
+      for (uint32_t s = H.pseg[p], s1; s < H.pseg[p + 1]; s = s1) {
+        s1 = chunk_end(s, H.pseg[p + 1]);
+        nck[p]++;
+        bsize[p] += align_up(rseg[s1] - rseg[s], kBAlign);
+      }
   });
   H.pchunk.assign((size_t)P + 1, 0);
-  for (uint32_t p = 0; p < P; ++p) H.pchunk[p + 1] = H.pchunk[p] + nck[p];
+  std::vector<uint64_t> bbase((size_t)P + 1, 0);
+  for (uint32_t p = 0; p < P; ++p) {
+    H.pchunk[p + 1] = H.pchunk[p] + nck[p];
+    bbase[p + 1] = bbase[p] + bsize[p];
+  }
   const uint32_t nchunks = H.pchunk[P];
+  const uint64_t nbel = bbase[P];
+  if (nbel + kChunkElems > 0xFFFFFFFFull) throw std::invalid_argument("too many elements");
   H.chunk_piece.assign((size_t)nchunks + 1, 0);
   H.chunk_elem.assign((size_t)nchunks + 1, 0);
   H.boff.assign((size_t)nchunks * (kBoxVox + 1), 0);
-  H.bpc.assign(nnz, 0);
-  H.bval.assign(nnz, 0.f);
+  H.bpc.assign(nbel + kChunkElems, 0);
+  H.bval.assign(nbel + kChunkElems, 0.f);
   parallel_for(TP, [&](int t) {
     std::vector<uint32_t> cur(kBoxVox + 1);
     for (uint32_t p = (uint32_t)t; p < P; p += (uint32_t)TP) {
       uint32_t k = H.pchunk[p];
+      uint64_t at0 = bbase[p];
       for (uint32_t s = H.pseg[p]; s < H.pseg[p + 1]; ++k) {
         const uint32_t s1 = chunk_end(s, H.pseg[p + 1]);
         const uint32_t e0 = rseg[s], e1 = rseg[s1];
         H.chunk_piece[k] = s;
-        H.chunk_elem[k] = e0;
+        H.chunk_elem[k] = (uint32_t)at0;
         // elements per voxel, then their offsets; elements placed voxel by
         // voxel, pieces (rows) ascending inside a voxel
         std::fill(cur.begin(), cur.end(), 0u);
@@ -307,16 +358,17 @@ void build_host_panels(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_
         for (uint32_t v = 0; v <= kBoxVox; ++v) off[v] = (uint16_t)cur[v];
         for (uint32_t q = s; q < s1; ++q)
           for (uint32_t e = rseg[q]; e < rseg[q + 1]; ++e) {
-            const uint32_t at = e0 + cur[rcol[e]]++;
+            const uint64_t at = at0 + cur[rcol[e]]++;
             H.bpc[at] = (uint16_t)(q - s);
             H.bval[at] = rval[e];
           }
+        at0 += align_up(e1 - e0, kBAlign);
         s = s1;
       }
     }
   });
   H.chunk_piece[nchunks] = (uint32_t)nseg;
-  H.chunk_elem[nchunks] = (uint32_t)nnz;
+  H.chunk_elem[nchunks] = (uint32_t)nbel;
 
   // launch order: boxes by decreasing element count (largest first)
   H.border.resize(P);

@@ -42,6 +42,8 @@ constexpr uint32_t kChunkPieces = 512;   // backward: pieces staged per chunk (a
 constexpr uint32_t kChunkElems = 8192;   // backward: elements staged per chunk (at most)
 constexpr uint32_t kFChunkPieces = 256;  // forward: pieces staged per chunk (at most)
 constexpr uint32_t kFChunkElems = 4096;  // forward: elements staged per chunk (at most)
+constexpr uint32_t kFAlign = 4;          // forward chunks start at multiples of 4 elements
+constexpr uint32_t kBAlign = 8;          // backward chunks start at multiples of 8 elements
 
 struct HostPanels {
   uint32_t rows = 0, cols = 0;
@@ -54,8 +56,8 @@ struct HostPanels {
   std::vector<uint32_t> pseg;         // [npanels+1] first piece of each box
   std::vector<uint32_t> fseg;         // [nseg+1] first element of each piece
   std::vector<uint32_t> fslot;        // [nseg] row-major slot of each piece
-  std::vector<float> fval;            // [nnz] values, piece-major
-  std::vector<uint16_t> fcol;         // [nnz] column inside the piece's box
+  std::vector<float> fval;            // values, piece-major, chunks aligned (zero gaps)
+  std::vector<uint16_t> fcol;         // column inside the piece's box (0 in the gaps)
   std::vector<uint32_t> rowslot;      // [rows+1] first slot of each device row
   std::vector<uint32_t> pfchunk;      // [npanels+1] first forward chunk of each box
   std::vector<uint32_t> fchunk_piece; // [nfchunks+1] first piece of each forward chunk
@@ -66,19 +68,24 @@ struct HostPanels {
   std::vector<uint32_t> chunk_elem;   // [nchunks+1] first element of each chunk
   std::vector<uint16_t> boff;         // [nchunks*(kBoxVox+1)] element offset of each
                                       //  in-box voxel, relative to the chunk's first
-  std::vector<uint16_t> bpc;          // [nnz] piece of each element, inside its chunk
-  std::vector<float> bval;            // [nnz] value of each element
+  std::vector<uint16_t> bpc;          // piece of each element inside its chunk, chunks
+                                      //  aligned (zero gaps)
+  std::vector<float> bval;            // value of each element
   std::vector<uint32_t> border;       // [npanels] launch order: decreasing element count
 };
 
 // Panel-order column of every file column (voxel x + nx*y + nx*ny*z,
 // helper_files_common/scannerconfig.cpp:30-34) for a grid of nx*ny*nz ==
 // cols voxels, with the first column of every box; without a grid (any
-// zero) columns keep their order and boxes are runs of kBoxVox columns.
-// boxgrid receives the number of boxes along x, y, z (box p at x = p % nbx,
-// y = (p / nbx) % nby, z = p / (nbx * nby)), or {P, 1, 1} without a grid.
-void panel_order(uint32_t cols, const uint32_t grid[3], std::vector<uint32_t>* pc_of_file,
-                 std::vector<uint32_t>* pstart, uint32_t boxgrid[3]);
+// zero) boxes are runs of kBoxVox columns. Inside a box the voxels are
+// numbered so that the backward's lane groups (8 consecutive positions each,
+// 8 groups per wave) get balanced work: with count[c] = elements of file
+// column c (may be null), the voxels sorted by count are dealt to the
+// groups in snake order, one position per group per round. boxgrid receives
+// the number of boxes along x, y, z, or {P, 1, 1} without a grid.
+void panel_order(uint32_t cols, const uint32_t grid[3], const uint32_t* count,
+                 std::vector<uint32_t>* pc_of_file, std::vector<uint32_t>* pstart,
+                 uint32_t boxgrid[3]);
 
 // Build the tiling from the device-order CSR (row pointers, {column, value
 // bits} records) and the panel-order column of every device column.
