@@ -127,8 +127,8 @@ __device__ __forceinline__ uint32_t bcast(uint32_t x) {
 // lane's sums: lane g8 = ei*L + q of its run of 8 reads element i + g8 from
 // LDS, and share ei takes elements i + ei*L + m, m < L, from its run of L
 // lanes; per element the lane reads its 16-byte quad of the operand row idx
-// from LDS (past e: weight 0). The reads go out in groups of up to four
-// before their multiply-adds.
+// from LDS (past e: weight 0). The reads go out in groups of N before
+// their multiply-adds.
 template <uint32_t L, uint32_t m, uint32_t n, uint32_t N>
 __device__ __forceinline__ void gather_rec(uint32_t idx, uint32_t vbits,
                                            const float4* __restrict__ op, uint32_t q, float4* g,
@@ -139,17 +139,17 @@ __device__ __forceinline__ void gather_rec(uint32_t idx, uint32_t vbits,
     gather_rec<L, m, n + 1, N>(idx, vbits, op, q, g, v);
   }
 }
-template <uint32_t L, uint32_t m = 0>
+template <uint32_t L, uint32_t N, uint32_t m = 0>
 __device__ __forceinline__ void batch8(uint32_t idx, float val, const float4* __restrict__ op,
                                        uint32_t q, float4& acc) {
-  constexpr uint32_t N = L < 4 ? L : 4;
+  static_assert(L % N == 0, "reads in flight divide the batch");
   if constexpr (m < L) {
     float4 g[N];
     float v[N];
     gather_rec<L, m, 0, N>(idx, __float_as_uint(val), op, q, g, v);
 #pragma unroll
     for (uint32_t n = 0; n < N; ++n) fma4(v[n], g[n], acc);
-    batch8<L, m + N>(idx, val, op, q, acc);
+    batch8<L, N, m + N>(idx, val, op, q, acc);
   }
 }
 
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(kBoxThreads) void k_box_fwd(
         const uint32_t my = i + g8, ic = min(my, e - 1);
         const uint32_t c = scol[ic];
         const float v = sval[ic];
-        batch8<L>(c, my < e ? v : 0.f, xs, q, acc);
+        batch8<L, L>(c, my < e ? v : 0.f, xs, q, acc);
       }
       acc = across_e<W>(acc);
       if (ei == 0) store_nt(part4 + (size_t)sslot[sg] * L + q, acc);
@@ -330,7 +330,7 @@ __global__ __launch_bounds__(kBoxThreads) __attribute__((amdgpu_waves_per_eu(4, 
         const uint32_t my = i + g8, ic = min(my, e - 1);
         const uint32_t pc = spc[ic];
         const float v = sval[ic];
-        batch8<L>(pc, my < e ? v : 0.f, rs, q, acc[j]);
+        batch8<L, (L < 4 ? L : 4)>(pc, my < e ? v : 0.f, rs, q, acc[j]);  // registers: 4
       }
     }
   }
