@@ -431,6 +431,16 @@ struct mlem_batch {
   const PanelFormat* P = nullptr;
   float* fpart = nullptr;
   float* upart = nullptr;
+  // Tiled blocks are independent within an iteration: they alternate between
+  // the matrix's stream (partials fpart / upart) and a second stream `side`
+  // (partials fpart2 / upart2), forked and joined by events, so that one
+  // block's bandwidth-bound epilogues overlap another block's LDS-bound
+  // products (a captured iteration holds the two branches). Lone frames run
+  // on the matrix's stream.
+  float* fpart2 = nullptr;
+  float* upart2 = nullptr;
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   // measured cost (ms) of one block-iteration at widths 4, 8, 16, 32
   // (autotuned batches; 0 = not measured)
   double wcost[4] = {0, 0, 0, 0};
@@ -464,6 +474,14 @@ struct mlem_batch {
     dfree(tiles_b);
     dfree(fpart);
     dfree(upart);
+    dfree(fpart2);
+    dfree(upart2);
+    if (side) {
+      (void)hipStreamSynchronize(side);
+      (void)hipStreamDestroy(side);
+    }
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
   }
 };
 
@@ -538,25 +556,28 @@ inline const uint32_t* file_to_block(const mlem_batch* B, int W) {
   return tiled_block(B, W) ? B->P->pc_of_file : B->mat->m.colmap;
 }
 
+// set 0: partials fpart / upart; set 1: fpart2 / upart2 (the side stream's)
 template <int W>
-void launch_tiled_block(mlem_batch* B, RhsBlock& b, hipStream_t s) {
+void launch_tiled_block(mlem_batch* B, RhsBlock& b, hipStream_t s, int set) {
   const DevMatrix& m = B->mat->m;
   const PanelFormat& P = *B->P;
-  launch_panel_fwd<W>(P, b.X, B->fpart, B->grid, s);
+  float* fpart = set ? B->fpart2 : B->fpart;
+  float* upart = set ? B->upart2 : B->upart;
+  launch_panel_fwd<W>(P, b.X, fpart, B->grid, s);
   hipLaunchKernelGGL((k_tiled_ratio<W>), dim3(B->grid), dim3(kBlock), 0, s, P.rowslot,
-                     P.rowpiece, B->fpart,
+                     P.rowpiece, fpart,
                      b.Y, b.R, m.rows, b.partials);
   hipLaunchKernelGGL((k_batch_stats_reduce<W>), dim3(1), dim3(kReduceLanes), 0, s, b.partials,
                      B->grid, b.counter, b.results, kRing);
-  launch_panel_bwd<W>(P, b.R, B->upart, B->grid, s);
-  hipLaunchKernelGGL((k_tiled_update<W>), dim3(B->grid), dim3(kBlock), 0, s, P.colslot, B->upart,
+  launch_panel_bwd<W>(P, b.R, upart, B->grid, s);
+  hipLaunchKernelGGL((k_tiled_update<W>), dim3(B->grid), dim3(kBlock), 0, s, P.colslot, upart,
                      P.norm, b.X, b.mask, m.cols);
 }
 
 template <int W>
-void launch_block(mlem_batch* B, RhsBlock& b, hipStream_t s) {
+void launch_block(mlem_batch* B, RhsBlock& b, hipStream_t s, int set = 0) {
   if (tiled_block(B, W)) {
-    launch_tiled_block<W>(B, b, s);
+    launch_tiled_block<W>(B, b, s, set);
     return;
   }
   const DevMatrix& m = B->mat->m;
@@ -586,17 +607,40 @@ void launch_single(mlem_batch* B, RhsBlock& b, hipStream_t s) {
   launch_bwd(m, B->mode, b.R, m.norm, b.X, true, s);
 }
 
-void enqueue_iteration(mlem_batch* B, hipStream_t s) {
-  for (auto& bp : B->blocks) {
-    switch (bp->W) {
-      case 1: launch_single(B, *bp, s); break;
-      case 4: launch_block<4>(B, *bp, s); break;
-      case 8: launch_block<8>(B, *bp, s); break;
-      case 16: launch_block<16>(B, *bp, s); break;
-      case 32: launch_block<32>(B, *bp, s); break;
-      default: raise(MLEM_EINVAL, "unsupported block width");
-    }
+void launch_any(mlem_batch* B, RhsBlock& b, hipStream_t s, int set) {
+  switch (b.W) {
+    case 1: launch_single(B, b, s); break;
+    case 4: launch_block<4>(B, b, s, set); break;
+    case 8: launch_block<8>(B, b, s, set); break;
+    case 16: launch_block<16>(B, b, s, set); break;
+    case 32: launch_block<32>(B, b, s, set); break;
+    default: raise(MLEM_EINVAL, "unsupported block width");
   }
+}
+
+void enqueue_iteration(mlem_batch* B, hipStream_t s) {
+  size_t tiled = 0;
+  for (auto& bp : B->blocks) tiled += tiled_block(B, bp->W) ? 1 : 0;
+  if (!B->side || tiled < 2) {
+    for (auto& bp : B->blocks) launch_any(B, *bp, s, 0);
+    check_launch("batched iteration launch");
+    return;
+  }
+  // fork: the side stream starts after everything before this iteration
+  MLEM_HIP(hipEventRecord(B->fork, s));
+  MLEM_HIP(hipStreamWaitEvent(B->side, B->fork, 0));
+  size_t k = 0;
+  for (auto& bp : B->blocks) {
+    if (!tiled_block(B, bp->W)) {
+      launch_any(B, *bp, s, 0);
+      continue;
+    }
+    const int set = (int)(k++ % 2);
+    launch_any(B, *bp, set ? B->side : s, set);
+  }
+  // join: the iteration ends when both streams have finished
+  MLEM_HIP(hipEventRecord(B->join, B->side));
+  MLEM_HIP(hipStreamWaitEvent(s, B->join, 0));
   check_launch("batched iteration launch");
 }
 
@@ -840,6 +884,13 @@ int mlem_batch_create(mlem_matrix* h, uint32_t frames, const int32_t* y, uint32_
       B->P = &ensure_panels(m);
       MLEM_HIP(hipMalloc(&B->fpart, 4ull * std::max<uint64_t>(B->P->nseg, 1) * B->wmax));
       MLEM_HIP(hipMalloc(&B->upart, 4ull * std::max<uint64_t>(B->P->ngroups, 1) * B->wmax));
+      if (frames >= 8) {  // room for two tiled blocks: a second stream
+        MLEM_HIP(hipMalloc(&B->fpart2, 4ull * std::max<uint64_t>(B->P->nseg, 1) * B->wmax));
+        MLEM_HIP(hipMalloc(&B->upart2, 4ull * std::max<uint64_t>(B->P->ngroups, 1) * B->wmax));
+        MLEM_HIP(hipStreamCreateWithFlags(&B->side, hipStreamNonBlocking));
+        MLEM_HIP(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming));
+        MLEM_HIP(hipEventCreateWithFlags(&B->join, hipEventDisableTiming));
+      }
     }
     // tiles (device-order pointers are on the host after open)
     prepare_tiles(m, m.stream);
