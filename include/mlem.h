@@ -329,11 +329,12 @@ typedef struct mlem_batch_options {
                        time, single frames included                       */
   int use_graph;    /* 1: replay each iteration from a captured hipGraph  */
   int mode;         /* MLEM_MODE_FAST (default): blocks run through the
-                       matrix's two-dimensional tiling (voxel panels; the
-                       gathered slice of every block stays in one XCD's L2),
-                       fp32 fused multiply-adds summed in a fixed order:
-                       deterministic, within 1e-4 of FP64 MLEM; lone frames
-                       on the single-image FAST kernels.
+                       matrix's box tiling (16x16x4-voxel boxes whose slice
+                       of a block, or whose voxel sums, stay in one
+                       workgroup's LDS or registers), fp32 fused
+                       multiply-adds summed in a fixed order: deterministic,
+                       at the fp32 floor of FP64 MLEM; lone frames on the
+                       single-image FAST kernels.
                        MLEM_MODE_EXACT: every frame's image is bit-identical
                        to the sequential driver's (per-frame sequential sums
                        over the CSR / CSC, lone frames on the EXACT kernels) */
@@ -382,7 +383,7 @@ MLEM_API int mlem_forward_k(mlem_matrix* m, uint32_t frames, const float* y, uin
 /* mlem_forward_k with the products' arithmetic chosen (mlem_forward_k is
  * MLEM_MODE_EXACT): MLEM_MODE_EXACT sums every frame's products one by one in
  * file element order (for integer counts x_K is bit-identical to mlem());
- * MLEM_MODE_FAST runs the products through the matrix's panel tiling (the
+ * MLEM_MODE_FAST runs the products through the matrix's box tiling (the
  * batched FAST kernels), fixed-order sums, deterministic for a given block
  * width. mlem_backward_k runs in the tape's mode. No reference counterpart. */
 MLEM_API int mlem_forward_k_mode(mlem_matrix* m, uint32_t frames, const float* y,
