@@ -8,10 +8,11 @@
 // copy is renumbered:
 //   voxels  -> 4x4x2 bricks (32 floats = one 128-byte line per brick), so a
 //              line of response finds several of its voxels in one cache line;
-//   rows    -> sinogram order (projected direction, radial offset, axial
-//              centre), so the rows one CU works on at a time are near-parallel
-//              neighbours that gather the same lines of x, and the rows a
-//              column gathers in the backprojection are clustered in r.
+//   rows    -> sinogram order (projected direction, then radial offset and
+//              axial centre in 4 x 4 blocks), so the rows one CU works on
+//              at a time are near-parallel neighbours that gather the same
+//              lines of x, and the rows a column gathers in the
+//              backprojection are clustered in r.
 // Only indices change: every row keeps its elements in file order and every
 // CSC column keeps its rows in ascending FILE order (the CSC is sorted from
 // the file-order CSR before rows are renumbered), so each row sum and column
@@ -56,8 +57,13 @@ __global__ __launch_bounds__(kBlock) void k_rowptr(const uint64_t* __restrict__ 
 }
 
 // Sinogram sort key of every row from its first and last voxel (projected
-// direction) and its voxel centroid (radial offset, axial centre):
-// 6 bits direction | 8 bits offset | 6 bits axial. Empty rows sort last.
+// direction) and its voxel centroid (radial offset, axial centre): 64
+// direction bins, 256 offset bins, 64 axial bins. Within a direction the
+// (offset, axial) plane is ordered in blocks of 4 x 4 bins, so the rows through
+// a compact group of voxels (a backprojection tile) fill fewer lines of r than
+// in offset-major order (pet: backprojection 3-4% faster, forward 2%):
+// 6 bits direction | 6 bits offset / 4 | 4 bits axial / 4 | 2 bits offset % 4 |
+// 2 bits axial % 4. Empty rows sort last.
 __global__ __launch_bounds__(kBlock) void k_row_keys(const uint32_t* __restrict__ rowptr,
                                                      const uint2* __restrict__ csr, uint32_t rows,
                                                      uint32_t nx, uint32_t ny, uint32_t nz,
@@ -105,7 +111,7 @@ __global__ __launch_bounds__(kBlock) void k_row_keys(const uint32_t* __restrict_
   const uint32_t pb = min((uint32_t)(div_rn(phi, pi) * 64.f), 63u);
   const uint32_t sb = (uint32_t)min(max(div_rn(s + smax, 2.f * smax) * 256.f, 0.f), 255.f);
   const uint32_t zb = (uint32_t)min(max(div_rn(mz, (float)nz) * 64.f, 0.f), 63.f);
-  keys[row] = (pb << 14) | (sb << 6) | zb;
+  keys[row] = (pb << 14) | ((sb >> 2) << 8) | ((zb >> 2) << 4) | ((sb & 3u) << 2) | (zb & 3u);
 }
 
 __global__ __launch_bounds__(kBlock) void k_iota(uint32_t* __restrict__ p, uint32_t n) {
