@@ -7,7 +7,8 @@
 // unrelated lines. At load time, when the voxel grid is known, the device
 // copy is renumbered:
 //   voxels  -> 4x4x2 bricks (32 floats = one 128-byte line per brick), so a
-//              line of response finds several of its voxels in one cache line;
+//              line of response finds several of its voxels in one cache line,
+//              the bricks grouped 8x8x2 so that consecutive tiles stay close;
 //   rows    -> sinogram order (projected direction, then radial offset and
 //              axial centre in 4 x 4 blocks), so the rows one CU works on
 //              at a time are near-parallel neighbours that gather the same
@@ -270,14 +271,23 @@ void launch_rowptr_from_rowidx(const uint64_t* rowidx, uint32_t rows, uint32_t* 
 
 std::vector<uint32_t> brick_colmap(uint32_t nx, uint32_t ny, uint32_t nz) {
   constexpr uint32_t BX = 4, BY = 4, BZ = 2, BV = BX * BY * BZ;
-  const uint64_t nbx = (nx + BX - 1) / BX, nby = (ny + BY - 1) / BY, nbz = (nz + BZ - 1) / BZ;
+  // bricks in raster order within groups of 8 x 8 x 2 bricks (32 x 32 x 4
+  // voxels), the groups in raster order: the tiles one XCD works through in
+  // turn stay in one compact region, whose rows share lines of r in L2
+  // (backprojection of the 192x192x64 matrix 5% faster than plain raster order)
+  constexpr uint64_t SX = 8, SY = 8, SZ = 2;
+  const uint64_t nsx = (nx + BX * SX - 1) / (BX * SX), nsy = (ny + BY * SY - 1) / (BY * SY),
+                 nsz = (nz + BZ * SZ - 1) / (BZ * SZ);
+  const uint64_t nbx = nsx * SX, nby = nsy * SY, nbz = nsz * SZ;
   const uint64_t cols = (uint64_t)nx * ny * nz;
   const uint64_t space = nbx * nby * nbz * BV;
   std::vector<uint32_t> key(cols);
   std::vector<uint8_t> used(space, 0);
   for (uint64_t c = 0; c < cols; ++c) {
     const uint64_t x = c % nx, y = (c / nx) % ny, z = c / ((uint64_t)nx * ny);
-    const uint64_t b = (x / BX) + nbx * ((y / BY) + nby * (z / BZ));
+    const uint64_t bx = x / BX, by = y / BY, bz = z / BZ;
+    const uint64_t b = ((bx / SX) + nsx * ((by / SY) + nsy * (bz / SZ))) * (SX * SY * SZ) +
+                       (bx % SX) + SX * ((by % SY) + SY * (bz % SZ));
     const uint64_t in = (x % BX) + BX * ((y % BY) + BY * (z % BZ));
     key[c] = (uint32_t)(b * BV + in);
     used[b * BV + in] = 1;
