@@ -476,7 +476,7 @@ __device__ __forceinline__ float long_segment_sum(const uint2* __restrict__ el,
   return sum;
 }
 
-// Elements of a dictionary tile are loaded NP at a time (NP = 2 or 4): NP
+// Elements of a dictionary tile are loaded NP at a time (NP = 4 or 8): NP
 // fp32 words, or NP 16-bit values (fp16 / bf16) packed two to a word, and NP
 // 16-bit dictionary positions. The 16-bit value forms widen exactly to fp32.
 template <int NW>
@@ -521,9 +521,12 @@ struct PosPack {
   __device__ static uint32_t get(W w, int q) { return (WS::at(w, q >> 1) >> (16 * (q & 1))) & 0xFFFFu; }
 };
 
-// Elements per stream load: 16-byte value loads and 8-byte position loads
-// (fp32) measured 1-3% faster than pairs.
-constexpr int kDictPack = 4;
+// Elements per stream load: fp32 values take 16-byte value loads and 8-byte
+// position loads (1-3% faster than pairs); 16-bit values take 8 elements per
+// load, 16 bytes of values and 16 of positions (on the 192x192x64 matrix the
+// fp16 forward 3% and the backward 8% faster than 4 per load).
+template <int VK>
+constexpr int dict_pack() { return VK == kF32 ? 4 : 8; }
 
 // Raw buffer resource over [base, base + bytes): 32-bit byte offsets held in
 // one VGPR (no 64-bit address per load), loads at or past `bytes` return 0.
@@ -636,7 +639,7 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
                                           const uint32_t* __restrict__ dict,
                                           const uint32_t* __restrict__ dmeta,
                                           const float* __restrict__ v, uint32_t nv, Epi& epi) {
-  constexpr int NP = kDictPack;
+  constexpr int NP = dict_pack<VK>();
   static_assert(EPT % NP == 0, "elements are loaded in packs of NP");
   using VP = ValPack<VK, NP>;
   using LP = PosPack<NP>;
@@ -649,8 +652,8 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
   // Large tiles store the products over the dictionary values; smaller tiles
   // keep two buffers.
   constexpr bool kAlias = T >= kDictAliasMin;
-  __shared__ __align__(16) float V[T];                  // gathered dictionary values
-  __shared__ __align__(16) float Pbuf[kAlias ? 2 : T];  // element products, tile order
+  __shared__ __align__(32) float V[T];                  // gathered dictionary values
+  __shared__ __align__(32) float Pbuf[kAlias ? 2 : T];  // element products, tile order
   float* P = kAlias ? V : Pbuf;
   __shared__ uint32_t S[NB + 1];
   __shared__ float Sum[NB];
