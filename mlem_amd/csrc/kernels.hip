@@ -52,8 +52,9 @@ __device__ __forceinline__ float dpp(float v) {
 }
 
 // Sum over each group of G consecutive lanes; every lane of the group gets
-// the (fixed-order) sum. G = 16 uses row DPP moves (no LDS traffic): swap
-// within pairs, within quads, mirror the half-rows, mirror the row.
+// the (fixed-order) sum. G = 16 and G = 8 use row DPP moves (no LDS traffic):
+// swap within pairs, within quads, mirror the half-rows (G = 8 stops here),
+// mirror the row.
 template <int G>
 __device__ __forceinline__ float group_sum(float v) {
   if constexpr (G == 16) {
@@ -61,6 +62,11 @@ __device__ __forceinline__ float group_sum(float v) {
     v = v + dpp<0x4E>(v);   // quad_perm [2,3,0,1]
     v = v + dpp<0x141>(v);  // row_half_mirror
     v = v + dpp<0x140>(v);  // row_mirror
+    return v;
+  } else if constexpr (G == 8) {
+    v = v + dpp<0xB1>(v);   // quad_perm [1,0,3,2]
+    v = v + dpp<0x4E>(v);   // quad_perm [2,3,0,1]
+    v = v + dpp<0x141>(v);  // row_half_mirror
     return v;
   } else {
 #pragma unroll
@@ -648,7 +654,7 @@ __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
   using F = float __attribute__((ext_vector_type(NP)));
   constexpr uint32_t T = NB * EPT;
   constexpr int H = EPT / NP;
-  constexpr int G = 16;
+  constexpr int G = 8;  // lanes per segment sum (pet rows of ~127 elements: 2-5% faster than 16)
   // Large tiles store the products over the dictionary values; smaller tiles
   // keep two buffers.
   constexpr bool kAlias = T >= kDictAliasMin;
