@@ -97,9 +97,13 @@ def test_recon_vs_fp64_dequantized(qtiny, mode):
     assert rel_linf(ft, ft64) < 1e-6
 
 
-@pytest.mark.parametrize("family", [("dict", 16), ("dict", 64), ("dict512", 32), ("tiles", 8),
-                                    ("groups", 16)],
-                         ids=lambda f: f"{f[0]}{f[1]}")
+# every dictionary tile size (16-bit tiles load 8 elements per lane and load,
+# so each size has its own number of packs per lane) and the other families
+FAMILIES_Q = [("dict", 8), ("dict", 16), ("dict", 24), ("dict", 32), ("dict", 48), ("dict", 64),
+              ("dict512", 16), ("dict512", 24), ("dict512", 32), ("tiles", 8), ("groups", 16)]
+
+
+@pytest.mark.parametrize("family", FAMILIES_Q, ids=lambda f: f"{f[0]}{f[1]}")
 @pytest.mark.parametrize("kind", KINDS)
 def test_small_families(kind, family):
     # 32^3 workload, 20 iterations, every kernel family on 16-bit values
@@ -116,6 +120,24 @@ def test_small_families(kind, family):
     err = rel_linf(x, xo)
     print(f"[precision] {kind} {family}: rel Linf vs fp64(original) {err:.3e}")
     assert err < 5e-2
+    A.close()
+
+
+@pytest.mark.parametrize("family", [f for f in FAMILIES_Q if f[0].startswith("dict")],
+                         ids=lambda f: f"{f[0]}{f[1]}")
+@pytest.mark.parametrize("kind", KINDS)
+def test_exact_projections_every_tile_size(kind, family):
+    # EXACT keeps its bitwise bar on every 16-bit dictionary tile size
+    pr = synth.make_problem("small")
+    m = pr.matrix
+    A = _open(m, precision=kind)
+    A.set_launch(family, family)
+    q, s = orc.quantize(m.rowidx, m.elements, kind)
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0.1, 3, m.cols).astype(np.float32)
+    assert np.array_equal(M.calcFwProj(A, x, "exact"), orc.fwproj(m.rowidx, q, x) * s)
+    r = rng.uniform(0.1, 3, m.rows).astype(np.float32)
+    assert np.array_equal(M.calcBkProj(A, r, "exact"), orc.bkproj(m.rowidx, q, r * s, m.cols))
     A.close()
 
 
