@@ -317,6 +317,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd(const uint32_t* __restrict__ col
   }
 }
 
+// ------------------------------------------------------------ tiled SpMV --
 // Sums of the products P over the segments [S[s], S[s+1]), s < nseg, of an
 // NB-lane block: G-lane group g takes segments g, g + NB/G, ... and writes
 // Sum[s]. Lanes read aligned 16-byte quads, the first and last masked to the
@@ -352,6 +353,22 @@ __device__ __forceinline__ void segment_sums(const float* __restrict__ P,
   }
 }
 
+// A tile is a run of whole segments (rows, or CSC columns) holding at most
+// T = 256*EPT elements and at most T segments, or one longer segment alone
+// (partitioned once at load time). A 256-lane block streams the tile's
+// element range with EPT independent coalesced loads per lane, issues all
+// its gathers, writes the products to LDS and sums every segment there with
+// 16-lane groups; the epilogue (ratio or multiplicative update) runs on the
+// segment sum. All of a tile's loads are in flight at once, which is what
+// keeps HBM busy: short segments no longer serialise load -> gather ->
+// reduce per wave.
+// Loads and gathers are unconditional (indices clamped to the tile's last
+// element; the duplicates are masked or never read), so the compiler keeps
+// all EPT element loads, then all EPT gathers, in flight together instead of
+// wrapping each in a branch that waits for the previous one. A tile holds at
+// most 256 segments: lane t owns segment t, fetches its end offset and its
+// epilogue operands (Epi::prefetch) together with the element loads, and
+// finishes it (Epi::finish) from the LDS sum, so no load sits on the tail.
 template <int EPT, class Epi>
 __device__ __forceinline__ void tile_spmv(const uint32_t* __restrict__ segptr,
                                           const uint2* __restrict__ el,
