@@ -651,7 +651,7 @@ __device__ __forceinline__ void load_dict_rounds(uint32_t rounds, const DictSlot
 }
 
 // EXACT: lane t adds segment t's products (each one fp32-rounded) one by one
-// in element order — the reference's sequential sum — instead of the 16-lane
+// in element order — the reference's sequential sum — instead of the 8-lane
 // tree; a segment longer than a tile is walked by one lane.
 template <int NB, int EPT, int VK, bool EXACT, class Epi>
 __device__ __forceinline__ void dict_spmv(const uint32_t* __restrict__ segptr,
