@@ -1235,14 +1235,10 @@ void fwd_dvx(const DevMatrix& m, const float* x, const int32_t* y, float* out, b
                        m.rscale, nullptr);
 }
 
-// EXACT sums run on the 512-lane dictionary kernels only (kDictWide).
 template <int NB, int EPT, int VK>
 void fwd_dv(const DevMatrix& m, const float* x, const int32_t* y, float* out, bool ratio,
             bool exact, const StatsState* st, hipStream_t s) {
-  if constexpr (NB == 512) {
-    if (exact) return fwd_dvx<NB, EPT, VK, true>(m, x, y, out, ratio, st, s);
-  }
-  if (exact) raise(MLEM_EINVAL, "EXACT dictionary kernels need the 512-lane family");
+  if (exact) return fwd_dvx<NB, EPT, VK, true>(m, x, y, out, ratio, st, s);
   fwd_dvx<NB, EPT, VK, false>(m, x, y, out, ratio, st, s);
 }
 
@@ -1276,10 +1272,7 @@ void bwd_dvx(const DevMatrix& m, const float* r, const float* norm, float* out, 
 template <int NB, int EPT, int VK>
 void bwd_dv(const DevMatrix& m, const float* r, const float* norm, float* out, bool update,
             bool exact, hipStream_t s) {
-  if constexpr (NB == 512) {
-    if (exact) return bwd_dvx<NB, EPT, VK, true>(m, r, norm, out, update, s);
-  }
-  if (exact) raise(MLEM_EINVAL, "EXACT dictionary kernels need the 512-lane family");
+  if (exact) return bwd_dvx<NB, EPT, VK, true>(m, r, norm, out, update, s);
   bwd_dvx<NB, EPT, VK, false>(m, r, norm, out, update, s);
 }
 
@@ -1496,10 +1489,10 @@ void launch_fwd(const DevMatrix& m, int mode, const float* x, const int32_t* y, 
     stream_fwd(m, mode, x, y, out, ratio, stats, s);
     return;
   }
-  // EXACT: the 512-lane dictionary kernels with per-lane ordered sums when
-  // they are configured, else one lane per row
+  // EXACT: the dictionary kernels (either block width) with per-lane ordered
+  // sums when they are configured, else one lane per row
   const bool exact = mode == MLEM_MODE_EXACT;
-  if (exact && m.fwd.kind != kDictWide) {
+  if (exact && !is_dict(m.fwd.kind)) {
     fwd_g<1, true>(m, x, y, out, ratio, stats, s);
   } else if (is_dict(m.fwd.kind)) {
     switch (dict_lanes(m.fwd.kind) * 1000 + m.fwd.ept) {
@@ -1542,7 +1535,7 @@ void launch_bwd(const DevMatrix& m, int mode, const float* r, const float* norm,
   }
   if (!m.csc) raise(MLEM_ESTATE, "matrix was opened without a CSC transpose");
   const bool exact = mode == MLEM_MODE_EXACT;
-  if (exact && m.bwd.kind != kDictWide) {
+  if (exact && !is_dict(m.bwd.kind)) {
     bwd_g<1, true>(m, r, norm, out, update, s);
   } else if (is_dict(m.bwd.kind)) {
     switch (dict_lanes(m.bwd.kind) * 1000 + m.bwd.ept) {
